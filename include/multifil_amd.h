@@ -1,0 +1,151 @@
+/*
+ * multifil_amd.h -- C ABI of the MI355X (gfx950) half-sarcomere step engine.
+ *
+ * The reference has no FFI: its hot path is pure-Python JAX.  The entry
+ * points below replace these reference functions, and are what a ctypes
+ * binding on the reference side would call:
+ *
+ *   mf_tables  <- transitions.py:99-190  expm_pade6_batch, with the generator
+ *                 builders transitions.py:434-513 (TM, 27 x 4x4) and
+ *                 transitions.py:632-922 (XB, 2*n_bins x 6x6; absorbing
+ *                 variant transitions.py:1029-1078)
+ *   mf_run     <- simulation.py:584-747  _run_sim_kernel: per instance the
+ *                 initial create_and_equilibrate (:633-642) and the lax.scan
+ *                 over timesteps of scan_fn (:679-721) = timestep.py:136-199
+ *                 (kinetics_step + solve_equilibrium) + axial_force_at_mline
+ *                 (forces.py:588-627) + compute_all_metrics (metrics_fn.py:
+ *                 154-417)
+ *   mf_run with n_steps = 1 and host-given uniforms is the teacher-forced
+ *                 single step, timestep.py:136-199 with random_values=
+ *                 (transitions.py:521,1086).
+ *
+ * Conventions: every pointer in mf_launch is DEVICE memory owned by the
+ * caller; calls are stream-ordered on the given hipStream_t (0 = null
+ * stream); a return value != 0 is an error whose text mf_last_error()
+ * returns.  No C++ exception crosses this boundary.  Calls that share
+ * buffers must not overlap.
+ */
+#ifndef MULTIFIL_AMD_H
+#define MULTIFIL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_ABI_VERSION 1
+#define MF_N_PARAMS 44      /* DynamicParams fields, params.py:73-553 order */
+#define MF_N_METRICS 52     /* metrics_fn.py:343-417 order                */
+#define MF_MAX_BINS 256     /* n_xb_bins upper bound                       */
+
+/* Solver modes. */
+#define MF_SOLVER_REFERENCE 0  /* FP32 Newton-CG, caps and tolerance floor of
+                                  solver.py:369-427,714-718                  */
+#define MF_SOLVER_REFINED   1  /* FP64 Newton residual + FP32 CG correction,
+                                  iterated to refine_tol                     */
+
+/* Instance flags. */
+#define MF_INST_INIT    1   /* run create_and_equilibrate before step 0     */
+#define MF_INST_DYNAMIC 2   /* dynamic lattice spacing (K_lat, nu)           */
+
+/* One lattice (SarcTopology.create output, packed).  All offsets are in
+ * elements into the float / int32 lattice blobs. */
+typedef struct mf_lattice {
+  int32_t n_thick, n_crowns, n_thin, n_sites;
+  int32_t n_xb_per_crown, n_faces, width, n_bins;
+  int64_t crown_offsets;    /* f32 [n_thick*n_crowns]                      */
+  int64_t crown_rests;      /* f32 [n_thick*n_crowns]                      */
+  int64_t binding_offsets;  /* f32 [n_thin*n_sites]                        */
+  int64_t binding_rests;    /* f32 [n_thin*n_sites]                        */
+  int64_t thin_rest_cum;    /* f32 [n_thin*n_sites] cumsum of binding_rests */
+  int64_t bin_edges;        /* f32 [n_bins+1]                              */
+  int64_t bin_centers;      /* f32 [n_bins]                                */
+  int64_t xb_thin;          /* i32 [X] target thin filament                */
+  int64_t xb_slot;          /* i32 [X] thin*n_faces + face                 */
+  int64_t xb_valid;         /* i32 [X] 0/1                                 */
+  int64_t face_sites;       /* i32 [n_thin*n_faces*width], padded like
+                               xb_to_site_indices (sarc_geometry.py:1480) */
+  int64_t face_count;       /* i32 [n_thin*n_faces]                        */
+  int64_t tm_prev, tm_next; /* i32 [n_thin*n_sites]                        */
+  float overlap_lo;         /* min(crown_offsets) - 13 nm (metrics_fn.py:108) */
+  float overlap_hi;         /* max(crown_offsets) + 13 nm (metrics_fn.py:109) */
+} mf_lattice;
+
+/* One simulated half-sarcomere in a launch. */
+typedef struct mf_instance {
+  int32_t lattice;     /* index into mf_launch.lattices                    */
+  int32_t n_steps;     /* steps to advance in this launch (may be 0)       */
+  int32_t t0;          /* global index of the first step: trace row, RNG
+                          counter, metrics row                             */
+  int32_t flags;       /* MF_INST_*                                        */
+  uint64_t seed;       /* Philox4x32-10 key                                */
+  int64_t state_off;   /* byte offset of the instance state in `state`     */
+  int64_t table_off;   /* float offset of the instance tables in `tables`  */
+  float K_lat;         /* dynamic spacing: lattice stiffness x n_thick     */
+  float nu;            /* dynamic spacing: Poisson exponent                */
+  int32_t pad0, pad1;
+} mf_instance;
+
+typedef struct mf_solver_cfg {
+  int32_t mode;        /* MF_SOLVER_*                                      */
+  int32_t n_newton;    /* Newton cap (params.py:655: 4)                    */
+  int32_t n_cg;        /* CG iterations per Newton step (params.py:656: 6) */
+  int32_t n_refine;    /* Newton cap in MF_SOLVER_REFINED                  */
+  float tol;           /* NaN: max(solver_tol, thick_k*1e-4, 0.25)         */
+  float refine_tol;    /* pN, MF_SOLVER_REFINED stopping residual          */
+  float dt;            /* ms                                               */
+  int32_t pad;
+} mf_solver_cfg;
+
+typedef struct mf_launch {
+  const mf_lattice* lattices;   /* [n_lattices]                            */
+  const float* lat_f32;         /* lattice float blob                      */
+  const int32_t* lat_i32;       /* lattice int blob                        */
+  const mf_instance* instances; /* [B]                                     */
+  const float* params;          /* [B][MF_N_PARAMS]                        */
+  const float* drivers;         /* [B][t_stride][3] = z_line, pCa, spacing */
+  const float* uniforms;        /* [B][t_stride][u_stride] (TM draws then
+                                   XB draws) or NULL for device Philox    */
+  uint8_t* state;               /* instance states, see mf_state_layout    */
+  float* tables;                /* per-instance transition tables          */
+  float* metrics;               /* [B][t_stride][MF_N_METRICS] or NULL     */
+  double* force;                /* [B][t_stride] FP64 M-line force or NULL */
+  int32_t n_lattices, B, t_stride, u_stride;
+  mf_solver_cfg cfg;
+  int32_t block;                /* threads per workgroup: 256, 512 or 1024 */
+  int32_t t_base;               /* added to every instance's t0            */
+  const mf_lattice* lattices_host; /* optional HOST copy of `lattices`; when
+                                   NULL the library reads the device copy  */
+} mf_launch;
+
+/* Byte layout of one instance state (all sections 8-byte aligned):
+ *   f64 pos[N]  (N = n_thick*n_crowns + n_thin*n_sites; thick then thin)
+ *   f64 spacing   (carried lattice spacing)
+ *   i32 xb_bound[X]   (-1 unbound)          X = total_xbs
+ *   i32 xb_key[X]     (2*n_bins table row of the last kinetics step)
+ *   i32 thin_bound[S] (-1 unbound)          S = n_thin*n_sites
+ *   u8  xb_state[X], u8 tm_state[S]
+ * mf_state_bytes returns the size for a lattice. */
+int64_t mf_state_bytes(int32_t n_thick, int32_t n_crowns, int32_t n_thin,
+                       int32_t n_sites, int32_t n_xb_per_crown);
+/* Floats of per-instance table storage for n_bins. */
+int64_t mf_table_floats(int32_t n_bins);
+
+int mf_abi_version(void);
+const char* mf_last_error(void);
+
+/* Transition tables for every instance from its drivers at row t0 (pCa,
+ * spacing) or, for MF_INST_DYNAMIC, its carried spacing. */
+int mf_tables(const mf_launch* L, void* stream);
+
+/* Advance every instance n_steps (one persistent workgroup per instance). */
+int mf_run(const mf_launch* L, void* stream);
+
+/* Dynamic LDS bytes and the workgroup size mf_run would use (diagnostics). */
+int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

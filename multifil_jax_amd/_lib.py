@@ -1,0 +1,94 @@
+"""ctypes binding of the gfx950 engine (include/multifil_amd.h).
+
+Loads ``libmultifil_amd.so`` from this package directory.  There is no CPU
+fallback: if the library is missing or fails to load, importing the engine
+raises, so a run can never silently use another implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmultifil_amd.so")
+
+MF_ABI_VERSION = 1
+MF_N_PARAMS = 44
+MF_N_METRICS = 52
+MF_MAX_BINS = 256
+MF_SOLVER_REFERENCE = 0
+MF_SOLVER_REFINED = 1
+MF_INST_INIT = 1
+MF_INST_DYNAMIC = 2
+
+EXPORTED = ("mf_abi_version", "mf_last_error", "mf_state_bytes", "mf_table_floats",
+            "mf_tables", "mf_run", "mf_run_lds_bytes")
+
+
+class MfLattice(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("n_thick", "n_crowns", "n_thin", "n_sites",
+                                         "n_xb_per_crown", "n_faces", "width", "n_bins")] + \
+               [(n, C.c_int64) for n in ("crown_offsets", "crown_rests", "binding_offsets",
+                                         "binding_rests", "thin_rest_cum", "bin_edges",
+                                         "bin_centers", "xb_thin", "xb_slot", "xb_valid",
+                                         "face_sites", "face_count", "tm_prev", "tm_next")] + \
+               [("overlap_lo", C.c_float), ("overlap_hi", C.c_float)]
+
+
+class MfInstance(C.Structure):
+    _fields_ = [("lattice", C.c_int32), ("n_steps", C.c_int32), ("t0", C.c_int32),
+                ("flags", C.c_int32), ("seed", C.c_uint64), ("state_off", C.c_int64),
+                ("table_off", C.c_int64), ("K_lat", C.c_float), ("nu", C.c_float),
+                ("pad0", C.c_int32), ("pad1", C.c_int32)]
+
+
+class MfSolverCfg(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("n_newton", C.c_int32), ("n_cg", C.c_int32),
+                ("n_refine", C.c_int32), ("tol", C.c_float), ("refine_tol", C.c_float),
+                ("dt", C.c_float), ("pad", C.c_int32)]
+
+
+class MfLaunch(C.Structure):
+    _fields_ = [("lattices", C.c_void_p), ("lat_f32", C.c_void_p), ("lat_i32", C.c_void_p),
+                ("instances", C.c_void_p), ("params", C.c_void_p), ("drivers", C.c_void_p),
+                ("uniforms", C.c_void_p), ("state", C.c_void_p), ("tables", C.c_void_p),
+                ("metrics", C.c_void_p), ("force", C.c_void_p),
+                ("n_lattices", C.c_int32), ("B", C.c_int32), ("t_stride", C.c_int32),
+                ("u_stride", C.c_int32), ("cfg", MfSolverCfg), ("block", C.c_int32),
+                ("t_base", C.c_int32), ("lattices_host", C.c_void_p)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the engine library once; raise if it is absent or mismatched."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (hipcc, gfx950). There is no CPU fallback.")
+    lib = C.CDLL(LIB_PATH)
+    lib.mf_abi_version.restype = C.c_int
+    lib.mf_last_error.restype = C.c_char_p
+    lib.mf_state_bytes.restype = C.c_int64
+    lib.mf_state_bytes.argtypes = [C.c_int32] * 5
+    lib.mf_table_floats.restype = C.c_int64
+    lib.mf_table_floats.argtypes = [C.c_int32]
+    for fn in ("mf_tables", "mf_run"):
+        getattr(lib, fn).restype = C.c_int
+        getattr(lib, fn).argtypes = [C.POINTER(MfLaunch), C.c_void_p]
+    lib.mf_run_lds_bytes.restype = C.c_int
+    lib.mf_run_lds_bytes.argtypes = [C.POINTER(MfLaunch), C.POINTER(C.c_int64)]
+    if lib.mf_abi_version() != MF_ABI_VERSION:
+        raise RuntimeError(f"ABI mismatch: library {lib.mf_abi_version()} != {MF_ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().mf_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed: {msg}")

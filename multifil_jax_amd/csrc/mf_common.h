@@ -1,0 +1,276 @@
+// Device helpers shared by the gfx950 kernels: parameter indices, Philox,
+// wave-level scans/reductions, the rate laws and the small-matrix expm.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/multifil_amd.h"
+
+namespace mf {
+
+// DynamicParams row layout (params.py:73-553 field order).
+enum Param {
+  P_THICK_K = 0, P_THIN_K, P_C_REST_W, P_C_REST_S, P_C_K_W, P_C_K_S, P_G_REST_W, P_G_REST_S,
+  P_G_K_W, P_G_K_S, P_TITIN_A, P_TITIN_B, P_TITIN_REST, P_TM_K01, P_TM_K12, P_TM_K23,
+  P_TM_K30, P_TM_KEQ01, P_TM_KEQ12, P_TM_KEQ23, P_TM_JC, P_TM_JM, P_R01, P_R12, P_R23,
+  P_R34, P_D23, P_D34, P_R40, P_R04, P_R05, P_U_DRX, P_U_LOOSE, P_U_T1, P_U_T2, P_SRX_K0,
+  P_SRX_KMAX, P_SRX_B, P_SRX_CA50, P_TEMP, P_SOLVER_TOL, P_PCA, P_ZLINE, P_LS
+};
+static_assert(P_LS + 1 == MF_N_PARAMS, "parameter row size");
+
+constexpr float RATE_CAP = 10000.0f;
+constexpr int N_SQUARINGS_CAP = 18;
+constexpr float N_TITIN = 6.0f;
+constexpr float XB_REACH = 13.0f;
+constexpr int WAVE = 64;
+
+// Per-instance table layout (floats).
+constexpr int TAB_HDR = 16;                 // pCa, d_kin, d_abs, valid, ...
+constexpr int TAB_TM = TAB_HDR;             // 27 configs x 4 rows x 4 cumsums
+constexpr int TAB_XB = TAB_TM + 27 * 16;    // 2*nb keys x 6 rows x 8 (6 cumsums + pad)
+__host__ __device__ inline int64_t tab_abs(int nb) { return TAB_XB + (int64_t)2 * nb * 48; }
+__host__ __device__ inline int64_t tab_floats(int nb) { return tab_abs(nb) + (int64_t)2 * nb * 4; }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Uniform number `idx` of (step, purpose) for an instance: word idx%4 of
+// Philox block idx/4, as (bits >> 9) * 2^-23.
+__device__ __forceinline__ float philox_uniform(uint64_t seed, int step, int purpose, int idx) {
+  const uint4 w = philox4x32_10(make_uint4((uint32_t)(idx >> 2), (uint32_t)step, (uint32_t)purpose, 0u),
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+  const int k = idx & 3;
+  const uint32_t bits = k == 0 ? w.x : k == 1 ? w.y : k == 2 ? w.z : w.w;
+  return (float)(bits >> 9) * 1.1920928955078125e-07f;
+}
+
+// ---------------------------------------------------------------------------
+// wave-level primitives (wave64)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+// inclusive prefix (lane 0 first)
+template <typename T>
+__device__ __forceinline__ T wave_prefix(T v, int lane) {
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const T t = __shfl_up(v, o, WAVE);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+// inclusive suffix (lane 63 first)
+template <typename T>
+__device__ __forceinline__ T wave_suffix(T v, int lane) {
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const T t = __shfl_down(v, o, WAVE);
+    if (lane + o < WAVE) v += t;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// rate laws (rate_functions.py) and generators (transitions.py)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ca_conc(float pCa) { return powf(10.0f, -pCa); }
+
+// Thermal energy in pN nm (transitions.py:708).
+__device__ __forceinline__ float kT_pNnm(float temp_c) {
+  return (1.3810e-23f * (temp_c + 273.15f)) * 1e21f;
+}
+
+// 4x4 TM generator of neighbour configuration cfg = 9 n2 + 3 n3 + n_closed
+// (transitions.py:434-513, rate_functions.py:165-311).
+__device__ inline void tm_generator(const float* prm, float pCa, int cfg, float Q[4][4]) {
+  const float n2 = (float)(cfg / 9), n3 = (float)((cfg / 3) % 3), nc = (float)(cfg % 3);
+  const float JC = prm[P_TM_JC], JM = prm[P_TM_JM];
+  const float h = JC * n2 + JM * n3 - 0.5f * (JC + JM) * nc;
+  const float fwd = expf(0.5f * h), bwd = expf(-0.5f * h);
+  const float ca = ca_conc(pCa);
+  const float k01 = prm[P_TM_K01] * ca * fwd;
+  const float k12 = prm[P_TM_K12] * fwd;
+  const float k23 = prm[P_TM_K23] * fwd;
+  const float k30 = prm[P_TM_K30];
+  const float k10 = (prm[P_TM_K01] / prm[P_TM_KEQ01]) * bwd;
+  const float k21 = (prm[P_TM_K12] / prm[P_TM_KEQ12]) * bwd;
+  const float k32 = (prm[P_TM_K23] / prm[P_TM_KEQ23]) * bwd;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Q[i][j] = 0.0f;
+  Q[0][0] = -k01; Q[0][1] = k01;
+  Q[1][0] = k10; Q[1][1] = -(k10 + k12); Q[1][2] = k12;
+  Q[2][1] = k21; Q[2][2] = -(k21 + k23); Q[2][3] = k23;
+  Q[3][0] = k30; Q[3][2] = k32; Q[3][3] = -(k30 + k32);
+}
+
+// 6x6 XB generator at axial offset x, spacing d, permissiveness perm
+// (transitions.py:632-804, rate_functions.py:318-753).
+__device__ inline void xb_generator(const float* prm, float pCa, float d, float x, float perm,
+                                    bool absorbing, float Q[6][6]) {
+  const float r = sqrtf(x * x + d * d);
+  const float th = atan2f(d, x);
+  const float kt = kT_pNnm(prm[P_TEMP]);
+  const float dgw = r - prm[P_G_REST_W], dcw = th - prm[P_C_REST_W];
+  const float dgs = r - prm[P_G_REST_S], dcs = th - prm[P_C_REST_S];
+  const float Ew = (0.5f * prm[P_G_K_W] * (dgw * dgw) + 0.5f * prm[P_C_K_W] * (dcw * dcw)) / kt;
+  const float Es = (0.5f * prm[P_G_K_S] * (dgs * dgs) + 0.5f * prm[P_C_K_S] * (dcs * dcs)) / kt;
+  const float Ediff = Ew - Es;
+  const float Udrx = prm[P_U_DRX];
+  const float Uloose = prm[P_U_LOOSE] + Ew;
+  const float Ut1 = prm[P_U_T1] + Es;
+  const float Ut2 = prm[P_U_T2] + Es;
+  const float fs = prm[P_G_K_S] * dgs + (1.0f / r) * prm[P_C_K_S] * dcs;
+  float r01 = perm * prm[P_R01] * expf(-Ew);
+  if (isnan(r01)) r01 = 0.0f;
+  float r10 = fminf(expf(logf(r01 + 0.005f) - (Udrx - Uloose)), RATE_CAP);
+  if (isnan(r10)) r10 = RATE_CAP;
+  float r12 = prm[P_R12] * expf(fminf(Ediff, 30.0f) / 2.0f);
+  if (isnan(r12)) r12 = 0.0f;
+  const float r21 = fminf(expf(logf(r12) - (Uloose - Ut1)), RATE_CAP);
+  const float r23 = fminf(prm[P_R23] * expf(-fs * prm[P_D23] / kt), RATE_CAP);
+  const float r32 = expf(fminf(logf(r23 + 1e-30f) + (Ut2 - Ut1), logf(RATE_CAP)));
+  const float r34 = fminf(prm[P_R34] * expf(fs * prm[P_D34] / kt), RATE_CAP);
+  const float r40 = prm[P_R40], r04 = prm[P_R04], r05 = prm[P_R05];
+  const float ca = ca_conc(pCa), b = prm[P_SRX_B];
+  const float cab = powf(ca, b);
+  const float r50 = prm[P_SRX_K0] +
+                    ((prm[P_SRX_KMAX] - prm[P_SRX_K0]) * cab) / (powf(prm[P_SRX_CA50], b) + cab);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Q[i][j] = 0.0f;
+  if (!absorbing) {  // rows 0 and 4 zeroed in the absorbing variant (transitions.py:1077)
+    Q[0][0] = -(r01 + r04 + r05); Q[0][1] = r01; Q[0][4] = r04; Q[0][5] = r05;
+    Q[4][0] = r40; Q[4][4] = -r40;
+  }
+  Q[1][0] = r10; Q[1][1] = -(r10 + r12); Q[1][2] = r12;
+  Q[2][1] = r21; Q[2][2] = -(r21 + r23); Q[2][3] = r23;
+  Q[3][2] = r32; Q[3][3] = -(r32 + r34); Q[3][4] = r34;
+  Q[5][0] = r50; Q[5][5] = -r50;
+}
+
+// ---------------------------------------------------------------------------
+// expm by scaling and squaring, the reference's variant (transitions.py:99-190)
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void matmul(const float A[N][N], const float B[N][N], float C[N][N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc = fmaf(A[i][k], B[k][j], acc);
+      C[i][j] = acc;
+    }
+}
+
+// A <- expm(A).  ||A/2^s||_inf <= 1/2; Pade-type rational approximant with the
+// reference's coefficients; (V-U) X = (V+U) by Gauss-Jordan without pivoting
+// (V-U = I - A_s/2 + O(A_s^2) is diagonally dominant for ||A_s|| <= 1/2);
+// min(s, 18) squarings; rows renormalised; NaN -> 1/N.
+template <int N>
+__device__ inline void expm_ref(float A[N][N]) {
+  float nrm = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    float rs = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N; ++j) rs += fabsf(A[i][j]);
+    nrm = fmaxf(nrm, rs);
+  }
+  int s = (int)ceilf(log2f(nrm / 0.5f + 1e-10f));
+  s = s < 0 ? 0 : s;
+  const float sc = ldexpf(1.0f, -s);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) A[i][j] *= sc;
+  const float b0 = 1.0f, b1 = 1.0f / 2.0f, b2 = 1.0f / 9.0f, b3 = 1.0f / 72.0f,
+              b4 = 1.0f / 1008.0f, b5 = 1.0f / 30240.0f, b6 = 1.0f / 1209600.0f;
+  float A2[N][N], A4[N][N], V[N][N], W[N][N];
+  matmul<N>(A, A, A2);
+  matmul<N>(A2, A2, A4);
+  matmul<N>(A4, A2, V);  // A6
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const float id = (i == j) ? 1.0f : 0.0f;
+      V[i][j] = b0 * id + b2 * A2[i][j] + b4 * A4[i][j] + b6 * V[i][j];
+      W[i][j] = b1 * id + b3 * A2[i][j] + b5 * A4[i][j];
+    }
+  matmul<N>(A, W, A2);  // U
+  // M = V - U (into W), R = V + U (into A)
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      W[i][j] = V[i][j] - A2[i][j];
+      A[i][j] = V[i][j] + A2[i][j];
+    }
+  // Gauss-Jordan: W X = A
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const float inv = 1.0f / W[k][k];
+#pragma unroll
+    for (int j = 0; j < N; ++j) { W[k][j] *= inv; A[k][j] *= inv; }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i == k) continue;
+      const float f = W[i][k];
+#pragma unroll
+      for (int j = 0; j < N; ++j) { W[i][j] = fmaf(-f, W[k][j], W[i][j]); A[i][j] = fmaf(-f, A[k][j], A[i][j]); }
+    }
+  }
+  const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
+  for (int q = 0; q < nsq; ++q) {
+    matmul<N>(A, A, A2);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) A[i][j] = A2[i][j];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    float rs = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N; ++j) rs += A[i][j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const float v = A[i][j] / rs;
+      A[i][j] = isnan(v) ? 1.0f / N : v;
+    }
+  }
+}
+
+}  // namespace mf

@@ -1,0 +1,1132 @@
+// gfx950 engine for the half-sarcomere timestep: transition tables and the
+// persistent per-instance step kernel, behind the C ABI of multifil_amd.h.
+//
+// One workgroup owns one instance for its whole run.  Node vectors of the
+// Newton-CG solve live in registers, chain-major: wave w owns filament chains
+// w, w+NW, ...; lane l owns contiguous nodes [l*c, l*c+c) of each, so the
+// preconditioner (a pair of prefix/suffix sums along each chain, the exact
+// inverse of the backbone tridiagonal) runs as wave scans without LDS traffic.
+// Only the CG search direction, positions, per-XB force/stiffness and the
+// binding bookkeeping are shared through LDS.
+#include "mf_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+using namespace mf;
+
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return 1; }
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return fail(std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// state layout
+// ---------------------------------------------------------------------------
+struct StateLayout {
+  int64_t pos, ls, xb_bound, xb_key, thin_bound, xb_state, tm, bytes;
+};
+__host__ __device__ inline int64_t al8(int64_t v) { return (v + 7) & ~int64_t(7); }
+__host__ __device__ inline StateLayout state_layout(int Tk, int nc, int Tn, int ns, int nx) {
+  const int64_t N = (int64_t)Tk * nc + (int64_t)Tn * ns, X = (int64_t)Tk * nc * nx,
+                S = (int64_t)Tn * ns;
+  StateLayout L;
+  L.pos = 0;
+  L.ls = L.pos + 8 * N;
+  L.xb_bound = L.ls + 8;
+  L.xb_key = al8(L.xb_bound + 4 * X);
+  L.thin_bound = al8(L.xb_key + 4 * X);
+  L.xb_state = al8(L.thin_bound + 4 * S);
+  L.tm = L.xb_state + X;
+  L.bytes = al8(L.tm + S);
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// transition tables: one matrix per thread
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_max) {
+  const int per = 27 + 4 * nb_max;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)(gid / per);
+  const int m = (int)(gid % per);
+  if (b >= L.B) return;
+  const mf_instance I = L.instances[b];
+  const mf_lattice G = L.lattices[I.lattice];
+  const int nb = G.n_bins;
+  const float* prm = L.params + (int64_t)b * MF_N_PARAMS;
+  const float* drv = L.drivers + ((int64_t)b * L.t_stride + I.t0 + L.t_base) * 3;
+  const float pCa = drv[1];
+  float d = drv[2];
+  if (I.flags & MF_INST_DYNAMIC) {
+    const StateLayout SL = state_layout(G.n_thick, G.n_crowns, G.n_thin, G.n_sites, G.n_xb_per_crown);
+    if (!(I.flags & MF_INST_INIT)) d = (float)*(const double*)(L.state + I.state_off + SL.ls);
+  }
+  float* tab = L.tables + I.table_off;
+  if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
+  if (m < 27) {
+    float Q[4][4];
+    tm_generator(prm, pCa, m, Q);
+    const float dt = L.cfg.dt;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
+    expm_ref<4>(Q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float c = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + m * 16 + i * 4 + j] = c; }
+    }
+    return;
+  }
+  int key = m - 27;
+  const bool absorbing = key >= 2 * nb_max;
+  if (absorbing) key -= 2 * nb_max;
+  if (key >= 2 * nb) return;
+  const float* centers = L.lat_f32 + G.bin_centers;
+  float Q[6][6];
+  xb_generator(prm, pCa, d, centers[key % nb], (float)(key / nb), absorbing, Q);
+  const float dt = L.cfg.dt;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Q[i][j] *= dt;
+  expm_ref<6>(Q);
+  if (!absorbing) {
+    float* row = tab + TAB_XB + (int64_t)key * 48;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float c = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) { c += Q[i][j]; row[i * 8 + j] = c; }
+      row[i * 8 + 6] = 0.0f;
+      row[i * 8 + 7] = 0.0f;
+    }
+  } else {
+    float* row = tab + tab_abs(nb) + (int64_t)key * 4;
+    row[0] = Q[3][4];
+    row[1] = Q[2][0];
+    row[2] = Q[3][0];
+    row[3] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// persistent step kernel
+// ---------------------------------------------------------------------------
+struct Dims {
+  int Tk, nc, Tn, ns, nx, W, nf, nb;
+  int nk, N, X, S, nch;
+};
+
+// Per-lane node slots (chain-major).
+template <int MS>
+struct Slots {
+  int node[MS];   // global DOF index, -1 if none
+  int li[MS];     // index along the chain
+  int chk[MS];    // ordinal of the chain among this wave's chains
+  int fil[MS];    // filament index within its type
+  bool thick[MS];
+  int nch;        // chains owned by this wave
+};
+
+// Workgroup-wide reductions through a ring of LDS scratch rows.
+template <int BS>
+struct Red {
+  static constexpr int NW = BS / WAVE;
+  static constexpr int RK = 8;          // values per reduction row
+  double* buf;                          // [4][RK][NW]
+  int ring;
+  double* big;                          // [MQ * (NW + 1)] metric reductions
+  __device__ double* row() { double* p = buf + (ring & 3) * RK * NW; ++ring; return p; }
+};
+
+template <int K, int BS>
+__device__ __forceinline__ void wg_sum(Red<BS>& rd, double (&v)[K], int lane, int wave) {
+  static_assert(K <= Red<BS>::RK, "too many values");
+  double* r = rd.row();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) r[k * Red<BS>::NW + wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < Red<BS>::NW; ++w) s += r[k * Red<BS>::NW + w];
+    v[k] = s;
+  }
+}
+template <int BS>
+__device__ __forceinline__ float wg_sum1f(Red<BS>& rd, float v, int lane, int wave) {
+  double* r = rd.row();
+  const float s = wave_sum(v);
+  if (lane == 0) r[wave] = (double)s;
+  __syncthreads();
+  float t = 0.0f;
+#pragma unroll
+  for (int w = 0; w < Red<BS>::NW; ++w) t += (float)r[w];
+  return t;
+}
+template <int BS>
+__device__ __forceinline__ double wg_max1(Red<BS>& rd, double v, int lane, int wave) {
+  double* r = rd.row();
+  const double s = wave_max(v);
+  if (lane == 0) r[wave] = s;
+  __syncthreads();
+  double t = r[0];
+#pragma unroll
+  for (int w = 1; w < Red<BS>::NW; ++w) t = fmax(t, r[w]);
+  return t;
+}
+
+// Metric partial sums (indices into one reduction row).  Entries from
+// Q_MAXK on are maxima (minima are reduced as maxima of the negation).
+enum MetricSlot {
+  Q_FORCE = 0, Q_SUMK, Q_SUMN, Q_E1, Q_DE1, Q_TIT, Q_DTIT, Q_WORKDX,
+  Q_XB0, Q_TM0 = Q_XB0 + 6, Q_OV = Q_TM0 + 4, Q_OV2, Q_OV3, Q_ATPC, Q_NEWLY, Q_ATPE, Q_TEAR,
+  Q_MAXK, Q_MAXN, Q_NMINK, Q_NMINN, MQ
+};
+
+// Two-level workgroup reduction of K values: entries [0, NSUM) summed, the
+// rest max-reduced.  Wave partials go to LDS, thread k folds value k over the
+// waves, and every thread gets a pointer to the K results in LDS.
+template <int K, int NSUM, int BS>
+__device__ __forceinline__ double* wg_reduce_mixed(Red<BS>& rd, double (&v)[K], int tid, int lane, int wave) {
+  constexpr int NW = BS / WAVE;
+  static_assert(K <= 40, "reduction scratch too small");
+  double* part = rd.big;       // [K][NW]
+  double* res = rd.big + K * NW;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double r = k < NSUM ? wave_sum(v[k]) : wave_max(v[k]);
+    if (lane == 0) part[k * NW + wave] = r;
+  }
+  __syncthreads();
+  if (tid < K) {
+    double a = part[tid * NW];
+    for (int w = 1; w < NW; ++w) a = tid < NSUM ? a + part[tid * NW + w] : fmax(a, part[tid * NW + w]);
+    res[tid] = a;
+  }
+  __syncthreads();
+  return res;
+}
+
+// Scan of slot values along every chain of this wave.  dir 0: inclusive
+// prefix (node order), 1: inclusive suffix.  Per-chain direction chosen by
+// `thick_dir`/`thin_dir`.
+template <int MS>
+__device__ __forceinline__ void chain_scan(float (&v)[MS], const Slots<MS>& sl, int lane, int thick_dir,
+                                           int thin_dir) {
+  for (int k = 0; k < sl.nch; ++k) {
+    bool th = false;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) th = (sl.chk[s] == k) ? sl.thick[s] : th;
+    const int dir = th ? thick_dir : thin_dir;
+    float acc = 0.0f;
+    if (dir == 0) {
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (sl.chk[s] == k) { acc += v[s]; v[s] = acc; }
+      const float incl = wave_prefix(acc, lane);
+      float ex = __shfl_up(incl, 1, WAVE);
+      if (lane == 0) ex = 0.0f;
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (sl.chk[s] == k) v[s] += ex;
+    } else {
+#pragma unroll
+      for (int s = MS - 1; s >= 0; --s)
+        if (sl.chk[s] == k) { acc += v[s]; v[s] = acc; }
+      const float incl = wave_suffix(acc, lane);
+      float ex = __shfl_down(incl, 1, WAVE);
+      if (lane == WAVE - 1) ex = 0.0f;
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (sl.chk[s] == k) v[s] += ex;
+    }
+  }
+}
+
+// z = M^-1 r with M the negated backbone tridiagonal (state.py:269-316):
+// thick (M-line anchored, tip free): z_i = sum_j (min(i,j)+1) r_j / k
+//   = prefix over i of (suffix sums of r) / k;
+// thin (free end at 0, Z-line anchored): z_i = sum_j (n - max(i,j)) r_j / k
+//   = suffix over i of (prefix sums of r) / k.
+template <int MS>
+__device__ __forceinline__ void precond(const float (&r)[MS], float (&z)[MS], const Slots<MS>& sl, int lane,
+                                        float inv_kk, float inv_kn) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) z[s] = sl.node[s] >= 0 ? r[s] : 0.0f;
+  chain_scan<MS>(z, sl, lane, 1, 0);
+  chain_scan<MS>(z, sl, lane, 0, 1);
+#pragma unroll
+  for (int s = 0; s < MS; ++s) z[s] *= sl.thick[s] ? inv_kk : inv_kn;
+}
+
+// All LDS arrays of one workgroup.
+template <bool REF64>
+struct Shared {
+  float* x32;        // [N]   positions (reference mode)
+  double* x64;       // [N]   positions (refined mode)
+  float* pv;         // [N]   CG direction
+  float* fx;         // [X]   XB axial force (reference mode)
+  double* fx64;      // [X]   XB axial force (refined mode)
+  float* kx;         // [X]   XB axial stiffness df/dx
+  short* xsite;      // [X]   DOF index of the bound site, -1 unbound
+  short* xnext;      // [X]   next XB in the site's bound list
+  int* xkey;         // [X]   table row of the last kinetics step
+  unsigned char* xst;  // [X] XB state
+  int* shead;        // [S]   first XB bound to the site, -1 none
+  int* tbound;       // [S]   thin.bound_to
+  unsigned char* tm; // [2][S] TM state (double buffered)
+  float* tmcum;      // [27*16]
+  float* edges;      // [nb+1]
+  float* prm;        // [MF_N_PARAMS]
+  double* red;       // reduction ring
+};
+
+template <int BS, bool REF64>
+__host__ __device__ inline int64_t lds_bytes(const Dims& d) {
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { const int64_t at = o; o = (o + bytes + 15) & ~int64_t(15); return at; };
+  if (REF64) take(8LL * d.N); else take(4LL * d.N);
+  take(4LL * d.N);
+  if (REF64) take(8LL * d.X); else take(4LL * d.X);
+  take(4LL * d.X); take(2LL * d.X); take(2LL * d.X); take(4LL * d.X); take(d.X);
+  take(4LL * d.S); take(4LL * d.S); take(2LL * d.S);
+  take(4 * 27 * 16); take(4LL * (d.nb + 1)); take(4 * MF_N_PARAMS);
+  take(8LL * (4 * 8 * (BS / WAVE) + 40 * (BS / WAVE + 1)));
+  return o;
+}
+
+template <int BS, bool REF64>
+__device__ inline Shared<REF64> carve(char* smem, const Dims& d) {
+  Shared<REF64> sh;
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { char* p = smem + o; o = (o + bytes + 15) & ~int64_t(15); return p; };
+  if (REF64) { sh.x64 = (double*)take(8LL * d.N); sh.x32 = nullptr; }
+  else { sh.x32 = (float*)take(4LL * d.N); sh.x64 = nullptr; }
+  sh.pv = (float*)take(4LL * d.N);
+  if (REF64) { sh.fx64 = (double*)take(8LL * d.X); sh.fx = nullptr; }
+  else { sh.fx = (float*)take(4LL * d.X); sh.fx64 = nullptr; }
+  sh.kx = (float*)take(4LL * d.X);
+  sh.xsite = (short*)take(2LL * d.X);
+  sh.xnext = (short*)take(2LL * d.X);
+  sh.xkey = (int*)take(4LL * d.X);
+  sh.xst = (unsigned char*)take(d.X);
+  sh.shead = (int*)take(4LL * d.S);
+  sh.tbound = (int*)take(4LL * d.S);
+  sh.tm = (unsigned char*)take(2LL * d.S);
+  sh.tmcum = (float*)take(4 * 27 * 16);
+  sh.edges = (float*)take(4LL * (d.nb + 1));
+  sh.prm = (float*)take(4 * MF_N_PARAMS);
+  sh.red = (double*)take(8LL * (4 * 8 * (BS / WAVE) + 40 * (BS / WAVE + 1)));
+  return sh;
+}
+
+template <bool REF64>
+__device__ __forceinline__ double posd(const Shared<REF64>& sh, int i) {
+  if constexpr (REF64) return sh.x64[i]; else return (double)sh.x32[i];
+}
+template <bool REF64>
+__device__ __forceinline__ float posf(const Shared<REF64>& sh, int i) {
+  if constexpr (REF64) return (float)sh.x64[i]; else return sh.x32[i];
+}
+
+// XB spring constants of a state (forces.py:378-394).
+struct Springs { float gk, gr, ck, cr; };
+__device__ __forceinline__ Springs springs(const float* prm, int st) {
+  const bool strong = (st == 2) || (st == 3);
+  Springs s;
+  s.gk = strong ? prm[P_G_K_S] : prm[P_G_K_W];
+  s.gr = strong ? prm[P_G_REST_S] : prm[P_G_REST_W];
+  s.ck = strong ? prm[P_C_K_S] : prm[P_C_K_W];
+  s.cr = strong ? prm[P_C_REST_S] : prm[P_C_REST_W];
+  return s;
+}
+
+// Axial force of a bound XB at offset xd (forces.py:366-399) and its
+// derivative d f / d xd (what jax.jvp produces at solver.py:413).
+template <typename T>
+__device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, float& kf) {
+  const T r = sqrt(xd * xd + d * d);
+  const T rs = r > T(1e-10) ? r : T(1e-10);
+  const T th = atan2(d, xd);
+  const T cs = xd / rs, sn = d / rs;
+  f = T(sp.gk) * (r - T(sp.gr)) * cs - (T(1) / rs) * T(sp.ck) * (th - T(sp.cr)) * sn;
+  const T r2 = rs * rs;
+  kf = (float)(T(sp.gk) * (T(1) - T(sp.gr) * d * d / (r2 * rs)) +
+               T(sp.ck) * d * (d + T(2) * xd * (th - T(sp.cr))) / (r2 * r2));
+}
+
+// Titin axial force on the tip crown and its derivative with respect to the
+// tip position (forces.py:136-157).
+template <typename T>
+__device__ __forceinline__ void titin(const float* prm, T tip, T z, T d, T& tt, float& ktt) {
+  const T ad = z - tip;
+  const T Lq = sqrt(ad * ad + d * d);
+  const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
+  const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
+  T F = T(prm[P_TITIN_A]) * exp(argc);
+  F = F > T(0) ? F : T(0);
+  const T cs = Lq > T(0) ? ad / Lq : T(0);
+  tt = T(N_TITIN) * F * cs;
+  const T Fp = (arg > T(-100) && arg < T(100)) ? T(prm[P_TITIN_B]) * F : T(0);
+  ktt = (float)(-T(N_TITIN) * (Fp * ad * ad / (Lq * Lq) + F * d * d / (Lq * Lq * Lq)));
+}
+
+// Residual at the current positions: per-XB forces/stiffnesses (XB phase),
+// then per-node passive + XB sums (node phase).  Returns max|F| over the
+// workgroup; f[] receives the lane's slot residuals, ktip the tip stiffness.
+template <int BS, int MS, bool REF64, typename T>
+__device__ __forceinline__ double residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                  const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
+                                  T (&f)[MS], float (&ktip)[MS], Red<BS>& rd, int tid, int lane, int wave) {
+  const float* prm = sh.prm;
+  for (int j = tid; j < dm.X; j += BS) {
+    const int site = sh.xsite[j];
+    T fj = T(0);
+    float kj = 0.0f;
+    const int st = sh.xst[j];
+    if (site >= 0 && st >= 1 && st <= 3) {
+      const int crown = j / dm.nx;
+      const T xd = T(posd<REF64>(sh, site)) - T(posd<REF64>(sh, crown));
+      xb_force<T>(springs(prm, st), xd, d, fj, kj);
+    }
+    if constexpr (REF64) sh.fx64[j] = (double)fj; else sh.fx[j] = (float)fj;
+    sh.kx[j] = kj;
+  }
+  __syncthreads();
+  const T kk = T(prm[P_THICK_K]), kn = T(prm[P_THIN_K]);
+  double mx = 0.0;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    f[s] = T(0);
+    ktip[s] = 0.0f;
+    const int n = sl.node[s];
+    if (n < 0) continue;
+    const int i = sl.li[s];
+    const T x0 = T(posd<REF64>(sh, n));
+    T net;
+    if (sl.thick[s]) {
+      const T xm = i > 0 ? T(posd<REF64>(sh, n - 1)) : T(0);
+      const T s_lo = (x0 - xm - T(rA[s])) * kk;
+      T s_hi;
+      if (i < dm.nc - 1) {
+        s_hi = (T(posd<REF64>(sh, n + 1)) - x0 - T(rB[s])) * kk;
+      } else {
+        titin<T>(prm, x0, zl, d, s_hi, ktip[s]);
+      }
+      net = s_hi - s_lo;
+      T xs = T(0);
+      for (int h = 0; h < dm.nx; ++h) {
+        const int j = n * dm.nx + h;
+        if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+      }
+      net += xs;
+    } else {
+      const T xp = i < dm.ns - 1 ? T(posd<REF64>(sh, n + 1)) : zl;
+      const T s_hi = (xp - x0 - T(rA[s])) * kn;
+      const T s_lo = i > 0 ? (x0 - T(posd<REF64>(sh, n - 1)) - T(rB[s])) * kn : T(0);
+      net = s_hi - s_lo;
+      T xs = T(0);
+      for (int j = sh.shead[n - dm.nk]; j >= 0; j = sh.xnext[j]) {
+        if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+      }
+      net -= xs;
+    }
+    f[s] = net;
+    mx = fmax(mx, fabs((double)net));
+  }
+  return wg_max1<BS>(rd, mx, lane, wave);
+}
+
+// -J p for the slot nodes (p read from LDS).
+template <int MS, bool REF64>
+__device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                        const float (&ktip)[MS], float kk, float kn, float (&Ap)[MS]) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    Ap[s] = 0.0f;
+    const int n = sl.node[s];
+    if (n < 0) continue;
+    const int i = sl.li[s];
+    const float p0 = sh.pv[n];
+    float net;
+    if (sl.thick[s]) {
+      const float pm = i > 0 ? sh.pv[n - 1] : 0.0f;
+      const float lo = kk * (p0 - pm);
+      const float hi = i < dm.nc - 1 ? kk * (sh.pv[n + 1] - p0) : ktip[s] * p0;
+      net = hi - lo;
+      for (int h = 0; h < dm.nx; ++h) {
+        const int j = n * dm.nx + h;
+        const int site = sh.xsite[j];
+        if (site >= 0) net += sh.kx[j] * (sh.pv[site] - p0);
+      }
+    } else {
+      const float pp = i < dm.ns - 1 ? sh.pv[n + 1] : 0.0f;
+      const float hi = kn * (pp - p0);
+      const float lo = i > 0 ? kn * (p0 - sh.pv[n - 1]) : 0.0f;
+      net = hi - lo;
+      for (int j = sh.shead[n - dm.nk]; j >= 0; j = sh.xnext[j]) net -= sh.kx[j] * (p0 - sh.pv[j / dm.nx]);
+    }
+    Ap[s] = -net;
+  }
+}
+
+// Unrolled preconditioned CG on A = -J from x0 = 0 (solver.py:329-366).  The
+// final iteration's unused preconditioner apply and rz reduction are skipped.
+template <int BS, int MS, bool REF64>
+__device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl, const float (&b)[MS],
+                           const float (&ktip)[MS], int n_cg, float (&x)[MS], Red<BS>& rd, int lane, int wave) {
+  const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
+  const float ikk = 1.0f / kk, ikn = 1.0f / kn;
+  float r[MS], z[MS], p[MS], Ap[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; }
+  float rz = 0.0f;
+  for (int it = 0;; ++it) {
+    // z = M^-1 r; rz = r.z; p = z + beta p (beta = 0 on the first pass)
+    precond<MS>(r, z, sl, lane, ikk, ikn);
+    float loc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) loc += r[s] * z[s];
+    const float rz_new = wg_sum1f<BS>(rd, loc, lane, wave);
+    const float beta = it == 0 ? 0.0f : rz_new / (rz + 1e-30f);
+    rz = rz_new;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      p[s] = it == 0 ? z[s] : z[s] + beta * p[s];
+      if (sl.node[s] >= 0) sh.pv[sl.node[s]] = p[s];
+    }
+    __syncthreads();
+    neg_jvp<MS, REF64>(sh, dm, sl, ktip, kk, kn, Ap);
+    loc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
+    const float pAp = wg_sum1f<BS>(rd, loc, lane, wave);
+    const float alpha = rz / (pAp + 1e-30f);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) { x[s] += alpha * p[s]; r[s] -= alpha * Ap[s]; }
+    if (it >= n_cg - 1) break;
+  }
+}
+
+// Equilibrium solve at spacing d, Z-line zl (solver.py:663-771, fixed spacing).
+// Reference mode: FP32 residual, stop at max|F| <= tol or n_newton.
+// Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or n_refine.
+template <int BS, int MS, bool REF64>
+__device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl, const float (&rA)[MS],
+                             const float (&rB)[MS], float zl, float d, const mf_solver_cfg& cfg, Red<BS>& rd,
+                             int tid, int lane, int wave, int& iters, double& resid) {
+  using T = typename std::conditional<REF64, double, float>::type;
+  T f[MS];
+  float ktip[MS], dx[MS], fb[MS];
+  float tol;
+  int cap;
+  if constexpr (REF64) {
+    tol = cfg.refine_tol;
+    cap = cfg.n_refine;
+  } else {
+    const float t0 = isnan(cfg.tol) ? sh.prm[P_SOLVER_TOL] : cfg.tol;
+    tol = fmaxf(t0, fmaxf(sh.prm[P_THICK_K] * 1e-4f, 0.25f));
+    cap = cfg.n_newton;
+  }
+  int it = 0;
+  double mx;
+  for (;;) {
+    mx = residual<BS, MS, REF64, T>(sh, dm, sl, rA, rB, T(zl), T(d), f, ktip, rd, tid, lane, wave);
+    if (!(mx > (double)tol && it < cap)) break;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
+    pcg<BS, MS, REF64>(sh, dm, sl, fb, ktip, cfg.n_cg, dx, rd, lane, wave);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      const int n = sl.node[s];
+      if (n < 0) continue;
+      const float v = isfinite(dx[s]) ? dx[s] : 0.0f;
+      if constexpr (REF64) sh.x64[n] += (double)v; else sh.x32[n] += v;
+    }
+    __syncthreads();
+    ++it;
+  }
+  iters = it;
+  resid = mx;
+}
+
+// Nearest reachable site of XB j (geometry.py:60-175): search from crown+13,
+// mask padding slots and sites at x <= 0, first minimum wins; returns the
+// site index along the thin filament and the float32 axial offset from the crown.
+template <bool REF64>
+__device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, const int32_t* face_sites,
+                                             const int32_t* face_count, int slot, int thin, int crown,
+                                             int& nearest, float& xdist) {
+  const float base = posf<REF64>(sh, crown);
+  const float head = base + XB_REACH;
+  const int cnt = face_count[slot];
+  const int32_t* cs = face_sites + (int64_t)slot * dm.W;
+  const int tb = dm.nk + thin * dm.ns;
+  float best = INFINITY;
+  int bj = 0;
+  for (int j = 0; j < dm.W; ++j) {
+    const float sp = posf<REF64>(sh, tb + cs[j]);
+    const float dd = fabsf(sp - head);
+    const bool ok = (j < cnt) && (sp > 0.0f);
+    if (ok && dd < best) { best = dd; bj = j; }
+  }
+  nearest = cs[bj];
+  xdist = posf<REF64>(sh, tb + nearest) - base;
+}
+
+__device__ __forceinline__ int bin_of(const float* edges, int nb, float x) {
+  // clip(digitize(x, edges) - 1, 0, nb - 1) (transitions.py:915-917): the
+  // number of edges <= x, minus one; NaN sorts after every edge.
+  if (isnan(x)) return nb - 1;
+  int lo = 0, hi = nb + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (edges[mid] <= x) lo = mid + 1; else hi = mid;
+  }
+  const int b = lo - 1;
+  return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
+template <int BS, int MS, int MX, bool REF64>
+__global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
+  extern __shared__ __align__(16) char smem[];
+  constexpr int NW = BS / WAVE;
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+  const int b = blockIdx.x;
+  const mf_instance I = L.instances[b];
+  if (I.n_steps <= 0 && !(I.flags & MF_INST_INIT)) return;
+  const mf_lattice G = L.lattices[I.lattice];
+  Dims dm;
+  dm.Tk = G.n_thick; dm.nc = G.n_crowns; dm.Tn = G.n_thin; dm.ns = G.n_sites; dm.nx = G.n_xb_per_crown;
+  dm.W = G.width; dm.nf = G.n_faces; dm.nb = G.n_bins;
+  dm.nk = dm.Tk * dm.nc; dm.N = dm.nk + dm.Tn * dm.ns; dm.X = dm.nk * dm.nx; dm.S = dm.Tn * dm.ns;
+  dm.nch = dm.Tk + dm.Tn;
+  Shared<REF64> sh = carve<BS, REF64>(smem, dm);
+  Red<BS> rd{sh.red, 0, sh.red + 4 * Red<BS>::RK * (BS / WAVE)};
+
+  const float* lf = L.lat_f32;
+  const int32_t* li = L.lat_i32;
+  const int32_t* xb_thin = li + G.xb_thin;
+  const int32_t* xb_slot = li + G.xb_slot;
+  const int32_t* xb_valid = li + G.xb_valid;
+  const int32_t* face_sites = li + G.face_sites;
+  const int32_t* face_count = li + G.face_count;
+  const int32_t* tm_prev = li + G.tm_prev;
+  const int32_t* tm_next = li + G.tm_next;
+  const float* prm_g = L.params + (int64_t)b * MF_N_PARAMS;
+  const float* drv = L.drivers + (int64_t)b * L.t_stride * 3;
+  float* tab = L.tables + I.table_off;
+  const StateLayout SL = state_layout(dm.Tk, dm.nc, dm.Tn, dm.ns, dm.nx);
+  uint8_t* stb = L.state + I.state_off;
+
+  // ---- static per-workgroup data ------------------------------------------
+  for (int i = tid; i < MF_N_PARAMS; i += BS) sh.prm[i] = prm_g[i];
+  for (int i = tid; i <= dm.nb; i += BS) sh.edges[i] = lf[G.bin_edges + i];
+  for (int i = tid; i < 27 * 16; i += BS) sh.tmcum[i] = tab[TAB_TM + i];
+
+  // chain-major slot map
+  Slots<MS> sl;
+  {
+    int base = 0, k = 0;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) { sl.node[s] = -1; sl.li[s] = 0; sl.chk[s] = -1; sl.fil[s] = 0; sl.thick[s] = false; }
+    for (int c = wave; c < dm.nch && k < 8; c += NW, ++k) {
+      const bool th = c < dm.Tk;
+      const int len = th ? dm.nc : dm.ns;
+      const int per = (len + WAVE - 1) / WAVE;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        const int j = s - base;
+        if (j >= 0 && j < per) {
+          const int loc = lane * per + j;
+          sl.chk[s] = k;
+          sl.thick[s] = th;
+          sl.fil[s] = th ? c : c - dm.Tk;
+          sl.li[s] = loc;
+          sl.node[s] = loc < len ? (th ? c * dm.nc + loc : dm.nk + (c - dm.Tk) * dm.ns + loc) : -1;
+        }
+      }
+      base += per;
+    }
+    sl.nch = k;
+  }
+  // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1}
+  float rA[MS], rB[MS], co[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    rA[s] = rB[s] = co[s] = 0.0f;
+    const int n = sl.node[s];
+    if (n < 0) continue;
+    const int i = sl.li[s];
+    if (sl.thick[s]) {
+      const float* cr = lf + G.crown_rests + (int64_t)sl.fil[s] * dm.nc;
+      rA[s] = cr[i];
+      rB[s] = i < dm.nc - 1 ? cr[i + 1] : 0.0f;
+      co[s] = lf[G.crown_offsets + n];
+    } else {
+      const float* br = lf + G.binding_rests + (int64_t)sl.fil[s] * dm.ns;
+      rA[s] = br[i];
+      rB[s] = i > 0 ? br[i - 1] : 0.0f;
+      co[s] = lf[G.thin_rest_cum + (n - dm.nk)];
+    }
+  }
+
+  // ---- load or realize the state -----------------------------------------
+  const bool init = (I.flags & MF_INST_INIT) != 0;
+  double ls_carry;
+  if (!init) {
+    const double* pos = (const double*)(stb + SL.pos);
+    for (int i = tid; i < dm.N; i += BS) {
+      if constexpr (REF64) sh.x64[i] = pos[i]; else sh.x32[i] = (float)pos[i];
+    }
+    const int32_t* xbb = (const int32_t*)(stb + SL.xb_bound);
+    const int32_t* xbk = (const int32_t*)(stb + SL.xb_key);
+    const uint8_t* xbs = stb + SL.xb_state;
+    for (int j = tid; j < dm.X; j += BS) {
+      const int bt = xbb[j];
+      sh.xsite[j] = (short)(bt >= 0 ? dm.nk + xb_thin[j] * dm.ns + bt : -1);
+      sh.xkey[j] = xbk[j];
+      sh.xst[j] = xbs[j];
+    }
+    const int32_t* tbd = (const int32_t*)(stb + SL.thin_bound);
+    const uint8_t* tms = stb + SL.tm;
+    for (int s = tid; s < dm.S; s += BS) { sh.tbound[s] = tbd[s]; sh.tm[s] = tms[s]; }
+    ls_carry = *(const double*)(stb + SL.ls);
+  } else {
+    // realize_state (state.py:191-262) at z0 = drivers[t0]
+    const float z0 = drv[(I.t0 + L.t_base) * 3 + 0];
+    for (int i = tid; i < dm.N; i += BS) {
+      const float v = i < dm.nk ? lf[G.crown_offsets + i] : z0 - lf[G.binding_offsets + (i - dm.nk)];
+      if constexpr (REF64) sh.x64[i] = (double)v; else sh.x32[i] = v;
+    }
+    for (int j = tid; j < dm.X; j += BS) { sh.xsite[j] = -1; sh.xst[j] = 0; }
+    for (int s = tid; s < dm.S; s += BS) { sh.tbound[s] = -1; sh.tm[s] = 0; }
+    ls_carry = (double)drv[(I.t0 + L.t_base) * 3 + 2];
+  }
+  for (int s = tid; s < dm.S; s += BS) sh.shead[s] = -1;
+  __syncthreads();
+
+  int tmc = 0;  // which TM buffer is current
+  const int S = dm.S, X = dm.X, nb = dm.nb;
+  const float* xbtab = tab + TAB_XB;
+  const float* abstab = tab + tab_abs(nb);
+  const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
+
+  // ---- time loop; step -1 is create_and_equilibrate (simulation.py:633-642)
+  for (int step = init ? -1 : 0; step < I.n_steps; ++step) {
+    const bool kin = step >= 0;
+    const int t = I.t0 + L.t_base + (kin ? step : 0);
+    const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
+    double xpre[MS];
+    double m_atp_c = 0, m_newly = 0, m_atp_exp = 0, m_tear = 0;
+    float z_solve, d_solve;
+    if (!kin) {
+      // Nearest update, then a solve against the parameter row's z_line and
+      // lattice_spacing (the reference's run() leaves those at their defaults).
+      for (int j = tid; j < dm.X; j += BS) {
+        int nr; float xd;
+        nearest_site<REF64>(sh, dm, face_sites, face_count, xb_slot[j], xb_thin[j], j / dm.nx, nr, xd);
+        sh.xkey[j] = bin_of(sh.edges, dm.nb, xd);  // tm == 0 everywhere: permissiveness 0
+      }
+      __syncthreads();
+      z_solve = sh.prm[P_ZLINE];
+      d_solve = sh.prm[P_LS];
+#pragma unroll
+      for (int s = 0; s < MS; ++s) xpre[s] = 0.0;
+    } else {
+    const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
+    const float dkin = lsd;
+    z_solve = zl;
+    d_solve = dkin;
+    // thin.axial += dz (simulation.py:683-687)
+    if (dz != 0.0f) {
+      for (int i = dm.nk + tid; i < dm.N; i += BS) {
+        if constexpr (REF64) sh.x64[i] += (double)dz; else sh.x32[i] += dz;
+      }
+      __syncthreads();
+    }
+    // pre-solve values kept for the metrics
+#pragma unroll
+    for (int s = 0; s < MS; ++s) xpre[s] = sl.node[s] >= 0 ? posd<REF64>(sh, sl.node[s]) : 0.0;
+
+    // ---- K1: TM transitions (transitions.py:516-625) + nearest sites ----------
+    const unsigned char* tmo = sh.tm + tmc * S;
+    unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
+    for (int s = tid; s < S; s += BS) {
+      const int thin = s / dm.ns, i = s - thin * dm.ns;
+      const int pv = tm_prev[s], nv = tm_next[s];
+      const int sp_ = tmo[thin * dm.ns + pv], sn_ = tmo[thin * dm.ns + nv];
+      const bool pr = pv != i, nr = nv != i;
+      int n2 = (pr && sp_ == 2) + (nr && sn_ == 2);
+      int n3 = (pr && sp_ == 3) + (nr && sn_ == 3);
+      int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
+      const int cfg = 9 * n2 + 3 * n3 + ncl;
+      const int st = tmo[s];
+      const float u = U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(I.seed, t, 0, s);
+      int ns;
+      if (st == 3 && sh.tbound[s] >= 0) {
+        ns = u < 1.0f ? 3 : 0;  // locked: [0, 0, 0, 1]
+      } else {
+        const float* c = sh.tmcum + cfg * 16 + st * 4;
+        ns = u < c[0] ? 0 : u < c[1] ? 1 : u < c[2] ? 2 : u < c[3] ? 3 : 0;
+      }
+      tmn[s] = (unsigned char)ns;
+      sh.shead[s] = -1;
+    }
+    int q_near[MX], q_bin[MX], q_old[MX], q_keyold[MX];
+    bool q_occ[MX];
+#pragma unroll
+    for (int q = 0; q < MX; ++q) {
+      const int j = tid + q * BS;
+      q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_keyold[q] = 0; q_occ[q] = false;
+      if (j < X) {
+        float xd;
+        nearest_site<REF64>(sh, dm, face_sites, face_count, xb_slot[j], xb_thin[j], j / dm.nx, q_near[q], xd);
+        q_bin[q] = bin_of(sh.edges, nb, xd);
+        q_occ[q] = sh.tbound[xb_thin[j] * dm.ns + q_near[q]] >= 0;
+        q_old[q] = sh.xst[j];
+        q_keyold[q] = sh.xkey[j];
+      }
+    }
+    __syncthreads();
+    tmc ^= 1;
+    const unsigned char* tmc_p = sh.tm + tmc * S;
+
+    // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
+#pragma unroll
+    for (int q = 0; q < MX; ++q) {
+      const int j = tid + q * BS;
+      if (j >= X) continue;
+      const int thin = xb_thin[j];
+      const int nsite = q_near[q];
+      const bool perm = (tmc_p[thin * dm.ns + nsite] == 3) && (xb_valid[j] != 0);
+      const int key = (perm ? nb : 0) + q_bin[q];
+      const int old = q_old[q];
+      const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(I.seed, t, 1, j);
+      const float4 c0 = *(const float4*)(xbtab + (int64_t)key * 48 + old * 8);
+      const float2 c1 = *(const float2*)(xbtab + (int64_t)key * 48 + old * 8 + 4);
+      int nw = u < c0.x ? 0 : u < c0.y ? 1 : u < c0.z ? 2 : u < c0.w ? 3 : u < c1.x ? 4 : u < c1.y ? 5 : 0;
+      const bool ob = old >= 1 && old <= 3, nbd = nw >= 1 && nw <= 3;
+      const bool binding = !ob && nbd, unbinding = ob && !nbd;
+      const bool can_bind = binding && perm && !q_occ[q];
+      int site = sh.xsite[j];
+      if (unbinding) {
+        if (site >= 0) sh.tbound[site - dm.nk] = -1;
+        site = -1;
+      }
+      if (can_bind) {
+        site = dm.nk + thin * dm.ns + nsite;
+        atomicMin((unsigned int*)&sh.tbound[site - dm.nk], (unsigned int)j);
+      }
+      if (binding && !can_bind) nw = 0;
+      sh.xsite[j] = (short)site;
+      sh.xst[j] = (unsigned char)nw;
+      sh.xkey[j] = key;
+      if (site >= 0) sh.xnext[j] = (short)atomicExch(&sh.shead[site - dm.nk], j);
+      // metric terms on (old, new) and the previous step's key (metrics_fn.py:286-335)
+      m_atp_c += (old == 3 && (nw == 4 || nw == 0)) ? 1.0 : 0.0;
+      m_newly += (old == 0 && nw == 1) ? 1.0 : 0.0;
+      if (old == 2 || old == 3) {
+        const float4 a = *(const float4*)(abstab + (int64_t)q_keyold[q] * 4);
+        if (old == 3) m_atp_exp += (double)a.x;
+        m_tear += (double)(old == 3 ? a.z : a.y);
+      }
+    }
+    __syncthreads();
+    }  // kinetics
+
+    // ---- equilibrium solve -------------------------------------------------
+    int iters;
+    double resid;
+    solve<BS, MS, REF64>(sh, dm, sl, rA, rB, z_solve, d_solve, L.cfg, rd, tid, lane, wave, iters, resid);
+    if (!kin) continue;
+    const double ls_new = (double)d_solve;
+    ls_carry = ls_new;
+    const unsigned char* tmc_p = sh.tm + tmc * S;
+
+    // ---- metrics (metrics_fn.py:154-417) -----------------------------------
+    if (L.metrics != nullptr || L.force != nullptr) {
+      const double kk = (double)sh.prm[P_THICK_K];
+      const double ta = sh.prm[P_TITIN_A], tbp = sh.prm[P_TITIN_B], trest = sh.prm[P_TITIN_REST];
+      // pass 1: per-thread partial sums, one two-level reduction
+      double v[MQ];
+#pragma unroll
+      for (int k = 0; k < MQ; ++k) v[k] = 0.0;
+      v[Q_MAXK] = v[Q_MAXN] = v[Q_NMINK] = v[Q_NMINN] = -INFINITY;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        const int n = sl.node[s];
+        if (n < 0) continue;
+        const double x = posd<REF64>(sh, n);
+        const double dd = x - (double)co[s];
+        if (sl.thick[s]) {
+          v[Q_SUMK] += dd;
+          v[Q_MAXK] = fmax(v[Q_MAXK], dd);
+          v[Q_NMINK] = fmax(v[Q_NMINK], -dd);
+          v[Q_WORKDX] += x - xpre[s];
+          if (sl.li[s] == 0) {
+            const double e1 = 0.5 * kk * dd * dd;
+            const double d0 = xpre[s] - (double)co[s];
+            v[Q_FORCE] += dd * kk;
+            v[Q_E1] += e1;
+            v[Q_DE1] += e1 - 0.5 * kk * d0 * d0;
+          }
+          if (sl.li[s] == dm.nc - 1) {
+            const double an = (double)zl - x, ao = (double)zl - xpre[s];
+            const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
+            const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
+            const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
+            v[Q_TIT] += en;
+            v[Q_DTIT] += en - eo;
+          }
+        } else {
+          v[Q_SUMN] += dd;
+          v[Q_MAXN] = fmax(v[Q_MAXN], dd);
+          v[Q_NMINN] = fmax(v[Q_NMINN], -dd);
+        }
+      }
+      for (int j = tid; j < X; j += BS) {
+        const int st = sh.xst[j];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[Q_XB0 + k] += st == k ? 1.0 : 0.0;
+      }
+      for (int s2 = tid; s2 < S; s2 += BS) {
+        const int st = tmc_p[s2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[Q_TM0 + k] += st == k ? 1.0 : 0.0;
+        const float x = posf<REF64>(sh, dm.nk + s2);
+        if (x >= G.overlap_lo && x <= G.overlap_hi && x > 0.0f) {
+          v[Q_OV] += 1.0;
+          v[Q_OV2] += st == 2 ? 1.0 : 0.0;
+          v[Q_OV3] += st == 3 ? 1.0 : 0.0;
+        }
+      }
+      v[Q_ATPC] = m_atp_c; v[Q_NEWLY] = m_newly; v[Q_ATPE] = m_atp_exp; v[Q_TEAR] = m_tear;
+      double* fin = wg_reduce_mixed<MQ, Q_MAXK, BS>(rd, v, tid, lane, wave);
+      const double mean_k = fin[Q_SUMK] / dm.nk, mean_n = fin[Q_SUMN] / dm.S;
+      // pass 2: population variances (jnp.std)
+      double w2[2] = {0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        const int n = sl.node[s];
+        if (n < 0) continue;
+        const double dd = posd<REF64>(sh, n) - (double)co[s];
+        if (sl.thick[s]) w2[0] += (dd - mean_k) * (dd - mean_k);
+        else w2[1] += (dd - mean_n) * (dd - mean_n);
+      }
+      double* fin2 = wg_reduce_mixed<2, 2, BS>(rd, w2, tid, lane, wave);
+      if (tid == 0) {
+        const double Xd = (double)X, Sd = (double)S;
+        const double force = fin[Q_FORCE];
+        const double nbound = fin[Q_XB0 + 1] + fin[Q_XB0 + 2] + fin[Q_XB0 + 3];
+        const double work = force * (fin[Q_WORKDX] / dm.nk);
+        const double ov = fin[Q_OV];
+        if (L.metrics) {
+          float* m = L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS;
+          m[0] = (float)force; m[1] = (float)resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
+          m[5] = (float)nbound;
+          for (int k = 0; k < 6; ++k) m[6 + k] = (float)fin[Q_XB0 + k];
+          m[12] = (float)(nbound / Xd);
+          for (int k = 0; k < 6; ++k) m[13 + k] = (float)(fin[Q_XB0 + k] / Xd);
+          for (int k = 0; k < 4; ++k) m[19 + k] = (float)fin[Q_TM0 + k];
+          for (int k = 0; k < 4; ++k) m[23 + k] = (float)(fin[Q_TM0 + k] / Sd);
+          m[27] = (float)(fin[Q_TM0 + 3] / Sd);
+          m[28] = (float)(fin[Q_OV2] / ov);
+          m[29] = (float)(fin[Q_OV3] / ov);
+          m[30] = (float)((fin[Q_OV2] + fin[Q_OV3]) / ov);
+          m[31] = (float)ov;
+          m[32] = (float)fin[Q_ATPC]; m[33] = (float)fin[Q_NEWLY];
+          m[34] = (float)mean_k; m[35] = (float)fin[Q_MAXK]; m[36] = (float)(-fin[Q_NMINK]);
+          m[37] = (float)sqrt(fin2[0] / dm.nk);
+          m[38] = (float)mean_n; m[39] = (float)fin[Q_MAXN]; m[40] = (float)(-fin[Q_NMINN]);
+          m[41] = (float)sqrt(fin2[1] / dm.S);
+          m[42] = (float)(fin[Q_E1] / dm.Tk); m[43] = (float)(fin[Q_DE1] / dm.Tk);
+          m[44] = (float)(fin[Q_TIT] / dm.Tk); m[45] = (float)(fin[Q_DTIT] / dm.Tk);
+          m[46] = (float)work; m[47] = (float)(work / (double)dm.nk);
+          m[48] = (float)fin[Q_ATPE]; m[49] = (float)fin[Q_TEAR];
+          m[50] = (float)(fin[Q_ATPE] > 0.01 ? work / fin[Q_ATPE] : 0.0);
+          m[51] = (float)iters;
+        }
+        if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
+      }
+    }
+  }
+
+  // ---- write the state back ------------------------------------------------
+  __syncthreads();
+  {
+    double* pos = (double*)(stb + SL.pos);
+    for (int i = tid; i < dm.N; i += BS) pos[i] = posd<REF64>(sh, i);
+    int32_t* xbb = (int32_t*)(stb + SL.xb_bound);
+    int32_t* xbk = (int32_t*)(stb + SL.xb_key);
+    uint8_t* xbs = stb + SL.xb_state;
+    for (int j = tid; j < dm.X; j += BS) {
+      const int site = sh.xsite[j];
+      xbb[j] = site >= 0 ? site - dm.nk - xb_thin[j] * dm.ns : -1;
+      xbk[j] = sh.xkey[j];
+      xbs[j] = sh.xst[j];
+    }
+    int32_t* tbd = (int32_t*)(stb + SL.thin_bound);
+    uint8_t* tms = stb + SL.tm;
+    const unsigned char* tcur = sh.tm + tmc * dm.S;
+    for (int s = tid; s < dm.S; s += BS) { tbd[s] = sh.tbound[s]; tms[s] = tcur[s]; }
+    if (tid == 0) *(double*)(stb + SL.ls) = ls_carry;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Variant {
+  int bs, ms, mx;
+  bool ref64;
+  const void* fn;
+  int64_t (*lds)(const Dims&);
+};
+
+template <int BS, int MS, int MX, bool R>
+Variant make_variant() {
+  return Variant{BS, MS, MX, R, (const void*)&run_kernel<BS, MS, MX, R>, &lds_bytes<BS, R>};
+}
+
+const Variant* variants(int& n) {
+  static const Variant v[] = {
+      make_variant<1024, 2, 1, false>(), make_variant<1024, 2, 1, true>(),
+      make_variant<1024, 8, 4, false>(), make_variant<1024, 8, 4, true>(),
+      make_variant<512, 4, 2, false>(),  make_variant<512, 4, 2, true>(),
+      make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
+      make_variant<256, 8, 4, false>(),  make_variant<256, 8, 4, true>(),
+      make_variant<256, 16, 16, false>(), make_variant<256, 16, 16, true>(),
+  };
+  n = (int)(sizeof(v) / sizeof(v[0]));
+  return v;
+}
+
+Dims dims_of(const mf_lattice& G) {
+  Dims d;
+  d.Tk = G.n_thick; d.nc = G.n_crowns; d.Tn = G.n_thin; d.ns = G.n_sites; d.nx = G.n_xb_per_crown;
+  d.W = G.width; d.nf = G.n_faces; d.nb = G.n_bins;
+  d.nk = d.Tk * d.nc; d.N = d.nk + d.Tn * d.ns; d.X = d.nk * d.nx; d.S = d.Tn * d.ns;
+  d.nch = d.Tk + d.Tn;
+  return d;
+}
+
+// Slots and XB registers a lattice needs at block size bs.
+void needs(const Dims& d, int bs, int& ms, int& mx) {
+  const int nw = bs / WAVE;
+  ms = 0;
+  for (int w = 0; w < nw; ++w) {
+    int m = 0;
+    for (int c = w; c < d.nch; c += nw) m += ((c < d.Tk ? d.nc : d.ns) + WAVE - 1) / WAVE;
+    ms = std::max(ms, m);
+  }
+  mx = (d.X + bs - 1) / bs;
+}
+
+int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant** out, int64_t* lds) {
+  int n;
+  const Variant* v = variants(n);
+  const bool r64 = L->cfg.mode == MF_SOLVER_REFINED;
+  for (int i = 0; i < n; ++i) {
+    if (v[i].ref64 != r64 || v[i].bs != L->block) continue;
+    bool ok = true;
+    int64_t need = 0;
+    for (const auto& G : lats) {
+      const Dims d = dims_of(G);
+      int ms, mx;
+      needs(d, v[i].bs, ms, mx);
+      int nchw = (d.nch + v[i].bs / WAVE - 1) / (v[i].bs / WAVE);
+      if (ms > v[i].ms || mx > v[i].mx || nchw > 8 || d.N > 32767) ok = false;
+      need = std::max(need, v[i].lds(d));
+    }
+    if (ok && need <= 160 * 1024) { *out = &v[i]; *lds = need; return 0; }
+  }
+  return fail("no kernel variant fits this lattice at block " + std::to_string(L->block) +
+              " (LDS or register slots exceeded)");
+}
+
+int fetch_lattices(const mf_launch* L, std::vector<mf_lattice>& lats) {
+  if (L->n_lattices <= 0 || L->B <= 0) return fail("empty launch");
+  lats.resize(L->n_lattices);
+  if (L->lattices_host) {
+    std::memcpy(lats.data(), L->lattices_host, sizeof(mf_lattice) * L->n_lattices);
+  } else {
+    HIPCHK(hipMemcpy(lats.data(), L->lattices, sizeof(mf_lattice) * L->n_lattices, hipMemcpyDeviceToHost));
+  }
+  for (const auto& G : lats) {
+    if (G.n_bins <= 0 || G.n_bins > MF_MAX_BINS) return fail("n_bins out of range");
+    if (G.n_xb_per_crown <= 0 || G.width <= 0) return fail("bad lattice");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mf_abi_version(void) { return MF_ABI_VERSION; }
+const char* mf_last_error(void) { return g_err.c_str(); }
+
+int64_t mf_state_bytes(int32_t n_thick, int32_t n_crowns, int32_t n_thin, int32_t n_sites,
+                       int32_t n_xb_per_crown) {
+  return state_layout(n_thick, n_crowns, n_thin, n_sites, n_xb_per_crown).bytes;
+}
+int64_t mf_table_floats(int32_t n_bins) { return tab_floats(n_bins); }
+
+int mf_tables(const mf_launch* L, void* stream) {
+  if (!L) return fail("null launch");
+  std::vector<mf_lattice> lats;
+  if (int e = fetch_lattices(L, lats)) return e;
+  int nb_max = 0;
+  for (const auto& G : lats) nb_max = std::max(nb_max, G.n_bins);
+  const int64_t total = (int64_t)L->B * (27 + 4 * nb_max);
+  const int grid = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(tables_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, *L, nb_max);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes) {
+  if (!L || !lds_bytes) return fail("null argument");
+  std::vector<mf_lattice> lats;
+  if (int e = fetch_lattices(L, lats)) return e;
+  const Variant* v;
+  return pick(L, lats, &v, lds_bytes);
+}
+
+int mf_run(const mf_launch* L, void* stream) {
+  if (!L) return fail("null launch");
+  if (L->block != 256 && L->block != 512 && L->block != 1024) return fail("block must be 256, 512 or 1024");
+  if (L->cfg.n_cg < 1 || L->cfg.n_cg > 64) return fail("n_cg out of range");
+  std::vector<mf_lattice> lats;
+  if (int e = fetch_lattices(L, lats)) return e;
+  const Variant* v;
+  int64_t lds;
+  if (int e = pick(L, lats, &v, &lds)) return e;
+  HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  void* args[] = {(void*)L};
+  HIPCHK(hipLaunchKernel(v->fn, dim3(L->B), dim3(v->bs), args, (size_t)lds, (hipStream_t)stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,347 @@
+"""Device-side batch engine: packs lattices and instances into flat HBM
+buffers and drives the gfx950 kernels through the C ABI.
+
+This replaces the reference's batch x time engine (multifil_jax/simulation.py:
+584-747, ``_run_sim_kernel``): one persistent workgroup per instance loops
+over timesteps on-chip instead of ``vmap(scan(timestep))``.
+
+Layout in HBM (per launch):
+  lattice blobs   float32 / int32 tables shared by every instance of a lattice
+  params          (B, 44) float32, DynamicParams field order
+  drivers         (B, T, 3) float32: z_line, pCa, lattice_spacing per step
+  state           per-instance byte records (positions f64, bindings, states)
+  tables          per-instance transition tables (cumulative P rows)
+  metrics         (B, T, 52) float32, plus the FP64 M-line force (B, T)
+PyTorch only provides device memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .params import N_PARAM_FIELDS
+
+_STATE_FIELDS = ("pos", "ls", "xb_bound", "xb_key", "thin_bound", "xb_state", "tm")
+
+
+def _al8(v: int) -> int:
+    return (v + 7) & ~7
+
+
+def state_layout(topo) -> dict:
+    """Byte offsets of an instance state record (multifil_amd.h)."""
+    N, X, S = topo.n_dof, topo.total_xbs, topo.n_thin * topo.n_sites
+    o = {"pos": 0}
+    o["ls"] = 8 * N
+    o["xb_bound"] = o["ls"] + 8
+    o["xb_key"] = _al8(o["xb_bound"] + 4 * X)
+    o["thin_bound"] = _al8(o["xb_key"] + 4 * X)
+    o["xb_state"] = _al8(o["thin_bound"] + 4 * S)
+    o["tm"] = o["xb_state"] + X
+    o["bytes"] = _al8(o["tm"] + S)
+    return o
+
+
+@dataclass
+class SolverConfig:
+    """Solver settings of one launch.
+
+    mode "reference": FP32 Newton-CG with the reference's caps (n_newton=4,
+    n_cg=6; params.py:655-656) and tolerance floor (solver.py:714-718).
+    mode "refined": the Newton residual in FP64 and the CG correction in FP32,
+    iterated until max|F| <= refine_tol pN (or n_refine iterations).
+    """
+    mode: str = "reference"
+    n_newton: int = 4
+    n_cg: int = 6
+    n_refine: int = 8
+    tol: Optional[float] = None
+    refine_tol: float = 1e-5
+    dt: float = 1.0
+
+    def ctype(self) -> _lib.MfSolverCfg:
+        if self.mode not in ("reference", "refined"):
+            raise ValueError(f"unknown solver mode {self.mode!r}")
+        return _lib.MfSolverCfg(
+            mode=_lib.MF_SOLVER_REFINED if self.mode == "refined" else _lib.MF_SOLVER_REFERENCE,
+            n_newton=self.n_newton, n_cg=self.n_cg, n_refine=self.n_refine,
+            tol=float("nan") if self.tol is None else float(self.tol),
+            refine_tol=float(self.refine_tol), dt=float(self.dt))
+
+
+class LatticePack:
+    """Flat float32 / int32 blobs holding one or more lattices."""
+
+    def __init__(self, topos: Sequence):
+        f32: List[np.ndarray] = []
+        i32: List[np.ndarray] = []
+        self.topos = list(topos)
+        descs = []
+        nf = ni = 0
+
+        def pf(a):
+            nonlocal nf
+            a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+            off = nf
+            f32.append(a)
+            nf += a.size
+            return off
+
+        def pi(a):
+            nonlocal ni
+            a = np.ascontiguousarray(a, dtype=np.int32).ravel()
+            off = ni
+            i32.append(a)
+            ni += a.size
+            return off
+
+        for t in self.topos:
+            if t.n_xb_bins > _lib.MF_MAX_BINS:
+                raise ValueError(f"n_xb_bins {t.n_xb_bins} > {_lib.MF_MAX_BINS}")
+            nfaces, width = t.n_faces_per_thin, t.max_sites_per_face
+            # Candidate lists per (thin, face), padded exactly like the
+            # reference's xb_to_site_indices (sarc_geometry.py:1475-1487).
+            fs = np.zeros((t.n_thin, nfaces, width), np.int32)
+            for i in range(t.n_thin):
+                for f in range(nfaces):
+                    n = int(t.n_sites_per_face[i, f])
+                    first = int(t.face_to_sites[i, f, 0]) if n > 0 else 0
+                    fs[i, f] = np.where(np.arange(width) < n, t.face_to_sites[i, f], first)
+            slot = t.xb_to_thin_id.astype(np.int64) * nfaces + t.xb_to_thin_face
+            # The face-table row of an XB must reproduce its own candidate row.
+            if not np.array_equal(fs.reshape(-1, width)[slot], t.xb_to_site_indices):
+                raise ValueError("xb_to_site_indices is not face_to_sites[thin, face]")
+            co = t.crown_offsets.astype(np.float32)
+            d = _lib.MfLattice(
+                n_thick=t.n_thick, n_crowns=t.n_crowns, n_thin=t.n_thin, n_sites=t.n_sites,
+                n_xb_per_crown=t.n_xb_per_crown, n_faces=nfaces, width=width, n_bins=t.n_xb_bins,
+                crown_offsets=pf(co), crown_rests=pf(t.crown_rests),
+                binding_offsets=pf(t.binding_offsets), binding_rests=pf(t.binding_rests),
+                thin_rest_cum=pf(np.cumsum(t.binding_rests.astype(np.float32), axis=1, dtype=np.float32)),
+                bin_edges=pf(t.xb_bin_edges), bin_centers=pf(t.xb_bin_centers),
+                xb_thin=pi(t.xb_to_thin_id), xb_slot=pi(slot), xb_valid=pi(t.xb_valid.astype(np.int32)),
+                face_sites=pi(fs), face_count=pi(t.n_sites_per_face),
+                tm_prev=pi(t.tm_prev_neighbor), tm_next=pi(t.tm_next_neighbor),
+                overlap_lo=float(np.float32(co.min()) - np.float32(13.0)),
+                overlap_hi=float(np.float32(co.max()) + np.float32(13.0)))
+            descs.append(d)
+        self.f32 = np.concatenate(f32) if f32 else np.zeros(1, np.float32)
+        self.i32 = np.concatenate(i32) if i32 else np.zeros(1, np.int32)
+        self.desc_host = (_lib.MfLattice * len(descs))(*descs)
+        self.desc_bytes = np.frombuffer(bytes(self.desc_host), dtype=np.uint8).copy()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Batch:
+    """B instances resident on one GPU, advanced by persistent launches.
+
+    lattice_ids: (B,) index into ``pack.topos`` per instance.
+    params: (B, 44) float32 parameter rows.
+    drivers: (B, T, 3) float32 (z_line, pCa, lattice_spacing per step).
+    seeds: (B,) uint64 Philox keys.
+    dynamic: None, or (K_lat (B,), nu (B,)) for dynamic lattice spacing.
+    """
+
+    def __init__(self, pack: LatticePack, lattice_ids, params, drivers, seeds,
+                 dynamic=None, device: str = "cuda", want_metrics: bool = True):
+        torch = _torch()
+        lib = _lib.load()
+        self.lib = lib
+        self.pack = pack
+        self.device = torch.device(device)
+        self.lattice_ids = np.asarray(lattice_ids, dtype=np.int32)
+        self.B = int(self.lattice_ids.size)
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.shape != (self.B, N_PARAM_FIELDS):
+            raise ValueError(f"params must be ({self.B}, {N_PARAM_FIELDS}), got {params.shape}")
+        drivers = np.ascontiguousarray(drivers, dtype=np.float32)
+        if drivers.ndim != 3 or drivers.shape[0] != self.B or drivers.shape[2] != 3:
+            raise ValueError(f"drivers must be ({self.B}, T, 3), got {drivers.shape}")
+        self.T = drivers.shape[1]
+        self.params_np = params
+        self.drivers_np = drivers
+        self.seeds = np.asarray(seeds, dtype=np.uint64)
+        self.dynamic = dynamic
+        if dynamic is not None:
+            raise NotImplementedError("dynamic lattice spacing is not available in this engine build")
+
+        topos = pack.topos
+        self.layouts = [state_layout(topos[i]) for i in self.lattice_ids]
+        sbytes = [lay["bytes"] for lay in self.layouts]
+        for i, lay in zip(self.lattice_ids, self.layouts):
+            t = topos[i]
+            got = lib.mf_state_bytes(t.n_thick, t.n_crowns, t.n_thin, t.n_sites, t.n_xb_per_crown)
+            if got != lay["bytes"]:
+                raise RuntimeError("state layout mismatch between host and library")
+        self.state_off = np.concatenate([[0], np.cumsum(sbytes)[:-1]]).astype(np.int64)
+        tfl = [int(lib.mf_table_floats(topos[i].n_xb_bins)) for i in self.lattice_ids]
+        self.table_off = np.concatenate([[0], np.cumsum(tfl)[:-1]]).astype(np.int64)
+
+        dev = self.device
+        self.d_lat = torch.from_numpy(pack.desc_bytes).to(dev)
+        self.d_f32 = torch.from_numpy(pack.f32).to(dev)
+        self.d_i32 = torch.from_numpy(pack.i32).to(dev)
+        self.d_params = torch.from_numpy(params).to(dev)
+        self.d_drivers = torch.from_numpy(drivers).to(dev)
+        self.d_state = torch.zeros(int(sum(sbytes)), dtype=torch.uint8, device=dev)
+        self.d_tables = torch.zeros(int(sum(tfl)), dtype=torch.float32, device=dev)
+        self.d_metrics = (torch.zeros((self.B, self.T, _lib.MF_N_METRICS), dtype=torch.float32, device=dev)
+                          if want_metrics else None)
+        self.d_force = torch.zeros((self.B, self.T), dtype=torch.float64, device=dev)
+        self._inst_cache = {}
+        self.d_inst = None
+        self.block = 1024
+        self.initialized = np.zeros(self.B, dtype=bool)
+        self._table_rows = None      # (B, 2) pCa / spacing the tables were built for
+        self._table_dt = None
+
+    # ------------------------------------------------------------------
+    def _instances(self, n_steps: int, init_mask):
+        """Device instance table for a launch (cached: per-launch step
+        offsets travel in mf_launch.t_base)."""
+        key = (int(n_steps), init_mask.tobytes())
+        hit = self._inst_cache.get(key)
+        if hit is None:
+            arr = (_lib.MfInstance * self.B)()
+            for b in range(self.B):
+                arr[b].lattice = int(self.lattice_ids[b])
+                arr[b].n_steps = int(n_steps)
+                arr[b].t0 = 0
+                arr[b].flags = _lib.MF_INST_INIT if init_mask[b] else 0
+                arr[b].seed = int(self.seeds[b])
+                arr[b].state_off = int(self.state_off[b])
+                arr[b].table_off = int(self.table_off[b])
+            buf = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+            hit = _torch().from_numpy(buf).to(self.device)
+            self._inst_cache[key] = hit
+        self.d_inst = hit
+
+    def _launch(self, cfg: SolverConfig, uniforms=None, u_stride=0) -> _lib.MfLaunch:
+        L = _lib.MfLaunch()
+        L.lattices = self.d_lat.data_ptr()
+        L.lat_f32 = self.d_f32.data_ptr()
+        L.lat_i32 = self.d_i32.data_ptr()
+        L.instances = self.d_inst.data_ptr()
+        L.params = self.d_params.data_ptr()
+        L.drivers = self.d_drivers.data_ptr()
+        L.uniforms = uniforms.data_ptr() if uniforms is not None else None
+        L.state = self.d_state.data_ptr()
+        L.tables = self.d_tables.data_ptr()
+        L.metrics = self.d_metrics.data_ptr() if self.d_metrics is not None else None
+        L.force = self.d_force.data_ptr()
+        L.n_lattices = len(self.pack.topos)
+        L.B = self.B
+        L.t_stride = self.T
+        L.u_stride = int(u_stride)
+        L.cfg = cfg.ctype()
+        L.block = int(self.block)
+        L.lattices_host = C.addressof(self.pack.desc_host)
+        return L
+
+    def _stream(self):
+        torch = _torch()
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def segments(self, t0: int, n: int) -> List[tuple]:
+        """Split [t0, t0+n) where any instance's pCa or spacing changes: the
+        transition tables are rebuilt between segments."""
+        pc = self.drivers_np[:, t0:t0 + n, 1:3]
+        change = np.any(np.any(pc[:, 1:] != pc[:, :-1], axis=2), axis=0)
+        cuts = [0] + [int(i) + 1 for i in np.nonzero(change)[0]] + [n]
+        return [(t0 + a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+
+    def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
+            block: Optional[int] = None, sync: bool = True):
+        """Advance every instance from step t0 by n_steps (the first call
+        equilibrates each instance before its first step, as the reference's
+        create_and_equilibrate does).
+
+        uniforms: optional device float32 tensor (B, T, S+X) of injected draws
+        (TM sites first, then XBs) instead of the device Philox stream.
+        """
+        cfg = cfg or SolverConfig()
+        if block is not None:
+            self.block = int(block)
+        if t0 < 0 or t0 + n_steps > self.T:
+            raise ValueError(f"steps [{t0}, {t0 + n_steps}) outside the driver trace (T={self.T})")
+        u_stride = 0
+        if uniforms is not None:
+            if tuple(uniforms.shape[:2]) != (self.B, self.T) or uniforms.dtype != _torch().float32:
+                raise ValueError("uniforms must be float32 (B, T, S+X)")
+            u_stride = uniforms.shape[2]
+            need = max(t.n_thin * t.n_sites + t.total_xbs for t in self.pack.topos)
+            if u_stride < need:
+                raise ValueError(f"uniforms last dim {u_stride} < {need}")
+            uniforms = uniforms.contiguous()
+        stream = self._stream()
+        segs = self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]
+        for (s0, n) in segs:
+            init = ~self.initialized
+            self._instances(n, init)
+            L = self._launch(cfg, uniforms, u_stride)
+            L.t_base = int(s0)
+            rows = self.drivers_np[:, s0, 1:3]
+            if self._table_rows is None or not np.array_equal(rows, self._table_rows) \
+                    or cfg.dt != self._table_dt:
+                _lib.check(self.lib.mf_tables(C.byref(L), stream), "mf_tables")
+                self._table_rows = rows.copy()
+                self._table_dt = cfg.dt
+            _lib.check(self.lib.mf_run(C.byref(L), stream), "mf_run")
+            self.initialized[:] = True
+        if sync:
+            _torch().cuda.synchronize(self.device)
+
+    # ------------------------------------------------------------------
+    def get_state(self, b: int) -> dict:
+        """Copy one instance state to host numpy arrays."""
+        t = self.pack.topos[self.lattice_ids[b]]
+        lay = self.layouts[b]
+        raw = self.d_state[self.state_off[b]:self.state_off[b] + lay["bytes"]].cpu().numpy()
+        N, X, S = t.n_dof, t.total_xbs, t.n_thin * t.n_sites
+        return {
+            "pos": raw[lay["pos"]:lay["pos"] + 8 * N].view(np.float64).copy(),
+            "ls": float(raw[lay["ls"]:lay["ls"] + 8].view(np.float64)[0]),
+            "xb_bound": raw[lay["xb_bound"]:lay["xb_bound"] + 4 * X].view(np.int32).copy(),
+            "xb_key": raw[lay["xb_key"]:lay["xb_key"] + 4 * X].view(np.int32).copy(),
+            "thin_bound": raw[lay["thin_bound"]:lay["thin_bound"] + 4 * S].view(np.int32).copy(),
+            "xb_state": raw[lay["xb_state"]:lay["xb_state"] + X].copy(),
+            "tm": raw[lay["tm"]:lay["tm"] + S].copy(),
+        }
+
+    def set_state(self, b: int, st: dict):
+        """Overwrite one instance state (marks it initialised)."""
+        torch = _torch()
+        t = self.pack.topos[self.lattice_ids[b]]
+        lay = self.layouts[b]
+        raw = np.zeros(lay["bytes"], np.uint8)
+        N, X, S = t.n_dof, t.total_xbs, t.n_thin * t.n_sites
+
+        def put(name, arr, dtype, n):
+            a = np.ascontiguousarray(arr, dtype=dtype).ravel()
+            if a.size != n:
+                raise ValueError(f"{name}: expected {n} values, got {a.size}")
+            raw[lay[name]:lay[name] + a.nbytes] = a.view(np.uint8)
+
+        put("pos", st["pos"], np.float64, N)
+        put("ls", [st["ls"]], np.float64, 1)
+        put("xb_bound", st["xb_bound"], np.int32, X)
+        put("xb_key", st["xb_key"], np.int32, X)
+        put("thin_bound", st["thin_bound"], np.int32, S)
+        put("xb_state", st["xb_state"], np.uint8, X)
+        put("tm", st["tm"], np.uint8, S)
+        self.d_state[self.state_off[b]:self.state_off[b] + lay["bytes"]] = torch.from_numpy(raw).to(self.device)
+        self.initialized[b] = True
+
+    def metrics(self) -> np.ndarray:
+        return self.d_metrics.cpu().numpy()
+
+    def force(self) -> np.ndarray:
+        return self.d_force.cpu().numpy()
