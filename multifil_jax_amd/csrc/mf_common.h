@@ -100,32 +100,39 @@ __device__ __forceinline__ T wave_suffix(T v, int lane) {
 // ---------------------------------------------------------------------------
 // rate laws (rate_functions.py) and generators (transitions.py)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float ca_conc(float pCa) { return powf(10.0f, -pCa); }
+// Rate laws and the expm are templated on the arithmetic type: the table
+// kernel evaluates them in double (the tables are built once per pCa /
+// spacing), so the stored float32 cumulative rows carry only their own
+// rounding.
+template <typename T>
+__device__ __forceinline__ T ca_conc(T pCa) { return pow(T(10), -pCa); }
 
 // Thermal energy in pN nm (transitions.py:708).
-__device__ __forceinline__ float kT_pNnm(float temp_c) {
-  return (1.3810e-23f * (temp_c + 273.15f)) * 1e21f;
+template <typename T>
+__device__ __forceinline__ T kT_pNnm(T temp_c) {
+  return (T(1.3810e-23) * (temp_c + T(273.15))) * T(1e21);
 }
 
 // 4x4 TM generator of neighbour configuration cfg = 9 n2 + 3 n3 + n_closed
 // (transitions.py:434-513, rate_functions.py:165-311).
-__device__ inline void tm_generator(const float* prm, float pCa, int cfg, float Q[4][4]) {
-  const float n2 = (float)(cfg / 9), n3 = (float)((cfg / 3) % 3), nc = (float)(cfg % 3);
-  const float JC = prm[P_TM_JC], JM = prm[P_TM_JM];
-  const float h = JC * n2 + JM * n3 - 0.5f * (JC + JM) * nc;
-  const float fwd = expf(0.5f * h), bwd = expf(-0.5f * h);
-  const float ca = ca_conc(pCa);
-  const float k01 = prm[P_TM_K01] * ca * fwd;
-  const float k12 = prm[P_TM_K12] * fwd;
-  const float k23 = prm[P_TM_K23] * fwd;
-  const float k30 = prm[P_TM_K30];
-  const float k10 = (prm[P_TM_K01] / prm[P_TM_KEQ01]) * bwd;
-  const float k21 = (prm[P_TM_K12] / prm[P_TM_KEQ12]) * bwd;
-  const float k32 = (prm[P_TM_K23] / prm[P_TM_KEQ23]) * bwd;
+template <typename T>
+__device__ inline void tm_generator(const float* prm, T pCa, int cfg, T Q[4][4]) {
+  const T n2 = (T)(cfg / 9), n3 = (T)((cfg / 3) % 3), nc = (T)(cfg % 3);
+  const T JC = prm[P_TM_JC], JM = prm[P_TM_JM];
+  const T h = JC * n2 + JM * n3 - T(0.5) * (JC + JM) * nc;
+  const T fwd = exp(T(0.5) * h), bwd = exp(T(-0.5) * h);
+  const T ca = ca_conc<T>(pCa);
+  const T k01 = T(prm[P_TM_K01]) * ca * fwd;
+  const T k12 = T(prm[P_TM_K12]) * fwd;
+  const T k23 = T(prm[P_TM_K23]) * fwd;
+  const T k30 = prm[P_TM_K30];
+  const T k10 = (T(prm[P_TM_K01]) / T(prm[P_TM_KEQ01])) * bwd;
+  const T k21 = (T(prm[P_TM_K12]) / T(prm[P_TM_KEQ12])) * bwd;
+  const T k32 = (T(prm[P_TM_K23]) / T(prm[P_TM_KEQ23])) * bwd;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) Q[i][j] = 0.0f;
+    for (int j = 0; j < 4; ++j) Q[i][j] = T(0);
   Q[0][0] = -k01; Q[0][1] = k01;
   Q[1][0] = k10; Q[1][1] = -(k10 + k12); Q[1][2] = k12;
   Q[2][1] = k21; Q[2][2] = -(k21 + k23); Q[2][3] = k23;
@@ -134,40 +141,42 @@ __device__ inline void tm_generator(const float* prm, float pCa, int cfg, float 
 
 // 6x6 XB generator at axial offset x, spacing d, permissiveness perm
 // (transitions.py:632-804, rate_functions.py:318-753).
-__device__ inline void xb_generator(const float* prm, float pCa, float d, float x, float perm,
-                                    bool absorbing, float Q[6][6]) {
-  const float r = sqrtf(x * x + d * d);
-  const float th = atan2f(d, x);
-  const float kt = kT_pNnm(prm[P_TEMP]);
-  const float dgw = r - prm[P_G_REST_W], dcw = th - prm[P_C_REST_W];
-  const float dgs = r - prm[P_G_REST_S], dcs = th - prm[P_C_REST_S];
-  const float Ew = (0.5f * prm[P_G_K_W] * (dgw * dgw) + 0.5f * prm[P_C_K_W] * (dcw * dcw)) / kt;
-  const float Es = (0.5f * prm[P_G_K_S] * (dgs * dgs) + 0.5f * prm[P_C_K_S] * (dcs * dcs)) / kt;
-  const float Ediff = Ew - Es;
-  const float Udrx = prm[P_U_DRX];
-  const float Uloose = prm[P_U_LOOSE] + Ew;
-  const float Ut1 = prm[P_U_T1] + Es;
-  const float Ut2 = prm[P_U_T2] + Es;
-  const float fs = prm[P_G_K_S] * dgs + (1.0f / r) * prm[P_C_K_S] * dcs;
-  float r01 = perm * prm[P_R01] * expf(-Ew);
-  if (isnan(r01)) r01 = 0.0f;
-  float r10 = fminf(expf(logf(r01 + 0.005f) - (Udrx - Uloose)), RATE_CAP);
-  if (isnan(r10)) r10 = RATE_CAP;
-  float r12 = prm[P_R12] * expf(fminf(Ediff, 30.0f) / 2.0f);
-  if (isnan(r12)) r12 = 0.0f;
-  const float r21 = fminf(expf(logf(r12) - (Uloose - Ut1)), RATE_CAP);
-  const float r23 = fminf(prm[P_R23] * expf(-fs * prm[P_D23] / kt), RATE_CAP);
-  const float r32 = expf(fminf(logf(r23 + 1e-30f) + (Ut2 - Ut1), logf(RATE_CAP)));
-  const float r34 = fminf(prm[P_R34] * expf(fs * prm[P_D34] / kt), RATE_CAP);
-  const float r40 = prm[P_R40], r04 = prm[P_R04], r05 = prm[P_R05];
-  const float ca = ca_conc(pCa), b = prm[P_SRX_B];
-  const float cab = powf(ca, b);
-  const float r50 = prm[P_SRX_K0] +
-                    ((prm[P_SRX_KMAX] - prm[P_SRX_K0]) * cab) / (powf(prm[P_SRX_CA50], b) + cab);
+template <typename T>
+__device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, bool absorbing,
+                                    T Q[6][6]) {
+  const T cap = T(RATE_CAP);
+  const T r = sqrt(x * x + d * d);
+  const T th = atan2(d, x);
+  const T kt = kT_pNnm<T>(T(prm[P_TEMP]));
+  const T dgw = r - T(prm[P_G_REST_W]), dcw = th - T(prm[P_C_REST_W]);
+  const T dgs = r - T(prm[P_G_REST_S]), dcs = th - T(prm[P_C_REST_S]);
+  const T Ew = (T(0.5) * T(prm[P_G_K_W]) * (dgw * dgw) + T(0.5) * T(prm[P_C_K_W]) * (dcw * dcw)) / kt;
+  const T Es = (T(0.5) * T(prm[P_G_K_S]) * (dgs * dgs) + T(0.5) * T(prm[P_C_K_S]) * (dcs * dcs)) / kt;
+  const T Ediff = Ew - Es;
+  const T Udrx = prm[P_U_DRX];
+  const T Uloose = T(prm[P_U_LOOSE]) + Ew;
+  const T Ut1 = T(prm[P_U_T1]) + Es;
+  const T Ut2 = T(prm[P_U_T2]) + Es;
+  const T fs = T(prm[P_G_K_S]) * dgs + (T(1) / r) * T(prm[P_C_K_S]) * dcs;
+  T r01 = perm * T(prm[P_R01]) * exp(-Ew);
+  if (isnan(r01)) r01 = T(0);
+  T r10 = fmin(exp(log(r01 + T(0.005)) - (Udrx - Uloose)), cap);
+  if (isnan(r10)) r10 = cap;
+  T r12 = T(prm[P_R12]) * exp(fmin(Ediff, T(30)) / T(2));
+  if (isnan(r12)) r12 = T(0);
+  const T r21 = fmin(exp(log(r12) - (Uloose - Ut1)), cap);
+  const T r23 = fmin(T(prm[P_R23]) * exp(-fs * T(prm[P_D23]) / kt), cap);
+  const T r32 = exp(fmin(log(r23 + T(1e-30)) + (Ut2 - Ut1), log(cap)));
+  const T r34 = fmin(T(prm[P_R34]) * exp(fs * T(prm[P_D34]) / kt), cap);
+  const T r40 = prm[P_R40], r04 = prm[P_R04], r05 = prm[P_R05];
+  const T ca = ca_conc<T>(pCa), b = prm[P_SRX_B];
+  const T cab = pow(ca, b);
+  const T r50 = T(prm[P_SRX_K0]) +
+                ((T(prm[P_SRX_KMAX]) - T(prm[P_SRX_K0])) * cab) / (pow(T(prm[P_SRX_CA50]), b) + cab);
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) Q[i][j] = 0.0f;
+    for (int j = 0; j < 6; ++j) Q[i][j] = T(0);
   if (!absorbing) {  // rows 0 and 4 zeroed in the absorbing variant (transitions.py:1077)
     Q[0][0] = -(r01 + r04 + r05); Q[0][1] = r01; Q[0][4] = r04; Q[0][5] = r05;
     Q[4][0] = r40; Q[4][4] = -r40;
@@ -181,56 +190,56 @@ __device__ inline void xb_generator(const float* prm, float pCa, float d, float 
 // ---------------------------------------------------------------------------
 // expm by scaling and squaring, the reference's variant (transitions.py:99-190)
 // ---------------------------------------------------------------------------
-template <int N>
-__device__ __forceinline__ void matmul(const float A[N][N], const float B[N][N], float C[N][N]) {
+template <int N, typename T>
+__device__ __forceinline__ void matmul(const T A[N][N], const T B[N][N], T C[N][N]) {
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      float acc = 0.0f;
+      T acc = T(0);
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc = fmaf(A[i][k], B[k][j], acc);
+      for (int k = 0; k < N; ++k) acc = fma(A[i][k], B[k][j], acc);
       C[i][j] = acc;
     }
 }
 
-// A <- expm(A).  ||A/2^s||_inf <= 1/2; Pade-type rational approximant with the
-// reference's coefficients; (V-U) X = (V+U) by Gauss-Jordan without pivoting
-// (V-U = I - A_s/2 + O(A_s^2) is diagonally dominant for ||A_s|| <= 1/2);
+// A <- expm(A).  ||A/2^s||_inf <= 1/2 with s from the float32 norm, as the
+// reference computes it; Pade-type rational approximant with the reference's
+// coefficients; (V-U) X = (V+U) by Gauss-Jordan without pivoting (V-U =
+// I - A_s/2 + O(A_s^2) is diagonally dominant for ||A_s|| <= 1/2);
 // min(s, 18) squarings; rows renormalised; NaN -> 1/N.
-template <int N>
-__device__ inline void expm_ref(float A[N][N]) {
-  float nrm = 0.0f;
+template <int N, typename T>
+__device__ inline void expm_ref(T A[N][N]) {
+  T nrm = T(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    float rs = 0.0f;
+    T rs = T(0);
 #pragma unroll
-    for (int j = 0; j < N; ++j) rs += fabsf(A[i][j]);
-    nrm = fmaxf(nrm, rs);
+    for (int j = 0; j < N; ++j) rs += fabs(A[i][j]);
+    nrm = fmax(nrm, rs);
   }
-  int s = (int)ceilf(log2f(nrm / 0.5f + 1e-10f));
+  int s = (int)ceilf(log2f((float)nrm / 0.5f + 1e-10f));
   s = s < 0 ? 0 : s;
-  const float sc = ldexpf(1.0f, -s);
+  const T sc = ldexp(T(1), -s);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) A[i][j] *= sc;
-  const float b0 = 1.0f, b1 = 1.0f / 2.0f, b2 = 1.0f / 9.0f, b3 = 1.0f / 72.0f,
-              b4 = 1.0f / 1008.0f, b5 = 1.0f / 30240.0f, b6 = 1.0f / 1209600.0f;
-  float A2[N][N], A4[N][N], V[N][N], W[N][N];
-  matmul<N>(A, A, A2);
-  matmul<N>(A2, A2, A4);
-  matmul<N>(A4, A2, V);  // A6
+  const T b0 = T(1), b1 = T(1) / T(2), b2 = T(1) / T(9), b3 = T(1) / T(72), b4 = T(1) / T(1008),
+          b5 = T(1) / T(30240), b6 = T(1) / T(1209600);
+  T A2[N][N], A4[N][N], V[N][N], W[N][N];
+  matmul<N, T>(A, A, A2);
+  matmul<N, T>(A2, A2, A4);
+  matmul<N, T>(A4, A2, V);  // A6
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const float id = (i == j) ? 1.0f : 0.0f;
+      const T id = (i == j) ? T(1) : T(0);
       V[i][j] = b0 * id + b2 * A2[i][j] + b4 * A4[i][j] + b6 * V[i][j];
       W[i][j] = b1 * id + b3 * A2[i][j] + b5 * A4[i][j];
     }
-  matmul<N>(A, W, A2);  // U
-  // M = V - U (into W), R = V + U (into A)
+  matmul<N, T>(A, W, A2);  // U
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -238,23 +247,22 @@ __device__ inline void expm_ref(float A[N][N]) {
       W[i][j] = V[i][j] - A2[i][j];
       A[i][j] = V[i][j] + A2[i][j];
     }
-  // Gauss-Jordan: W X = A
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    const float inv = 1.0f / W[k][k];
+    const T inv = T(1) / W[k][k];
 #pragma unroll
     for (int j = 0; j < N; ++j) { W[k][j] *= inv; A[k][j] *= inv; }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       if (i == k) continue;
-      const float f = W[i][k];
+      const T f = W[i][k];
 #pragma unroll
-      for (int j = 0; j < N; ++j) { W[i][j] = fmaf(-f, W[k][j], W[i][j]); A[i][j] = fmaf(-f, A[k][j], A[i][j]); }
+      for (int j = 0; j < N; ++j) { W[i][j] = fma(-f, W[k][j], W[i][j]); A[i][j] = fma(-f, A[k][j], A[i][j]); }
     }
   }
   const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
   for (int q = 0; q < nsq; ++q) {
-    matmul<N>(A, A, A2);
+    matmul<N, T>(A, A, A2);
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -262,13 +270,13 @@ __device__ inline void expm_ref(float A[N][N]) {
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    float rs = 0.0f;
+    T rs = T(0);
 #pragma unroll
     for (int j = 0; j < N; ++j) rs += A[i][j];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const float v = A[i][j] / rs;
-      A[i][j] = isnan(v) ? 1.0f / N : v;
+      const T v = A[i][j] / rs;
+      A[i][j] = isnan(v) ? T(1) / T(N) : v;
     }
   }
 }

@@ -75,19 +75,19 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   float* tab = L.tables + I.table_off;
   if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
   if (m < 27) {
-    float Q[4][4];
-    tm_generator(prm, pCa, m, Q);
-    const float dt = L.cfg.dt;
+    double Q[4][4];
+    tm_generator<double>(prm, (double)pCa, m, Q);
+    const double dt = L.cfg.dt;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
-    expm_ref<4>(Q);
+    expm_ref<4, double>(Q);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      float c = 0.0f;
+      double c = 0.0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + m * 16 + i * 4 + j] = c; }
+      for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + m * 16 + i * 4 + j] = (float)c; }
     }
     return;
   }
@@ -96,29 +96,30 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   if (absorbing) key -= 2 * nb_max;
   if (key >= 2 * nb) return;
   const float* centers = L.lat_f32 + G.bin_centers;
-  float Q[6][6];
-  xb_generator(prm, pCa, d, centers[key % nb], (float)(key / nb), absorbing, Q);
-  const float dt = L.cfg.dt;
+  double Q[6][6];
+  xb_generator<double>(prm, (double)pCa, (double)d, (double)centers[key % nb], (double)(key / nb),
+                       absorbing, Q);
+  const double dt = L.cfg.dt;
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = 0; j < 6; ++j) Q[i][j] *= dt;
-  expm_ref<6>(Q);
+  expm_ref<6, double>(Q);
   if (!absorbing) {
     float* row = tab + TAB_XB + (int64_t)key * 48;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-      float c = 0.0f;
+      double c = 0.0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) { c += Q[i][j]; row[i * 8 + j] = c; }
+      for (int j = 0; j < 6; ++j) { c += Q[i][j]; row[i * 8 + j] = (float)c; }
       row[i * 8 + 6] = 0.0f;
       row[i * 8 + 7] = 0.0f;
     }
   } else {
     float* row = tab + tab_abs(nb) + (int64_t)key * 4;
-    row[0] = Q[3][4];
-    row[1] = Q[2][0];
-    row[2] = Q[3][0];
+    row[0] = (float)Q[3][4];
+    row[1] = (float)Q[2][0];
+    row[2] = (float)Q[3][0];
     row[3] = 0.0f;
   }
 }

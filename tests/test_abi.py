@@ -1,0 +1,70 @@
+"""The C-ABI library loads and exports every symbol include/multifil_amd.h
+declares; ctypes mirrors match the C struct layouts; pure-host entry points
+answer without a GPU."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "multifil_amd.h")
+
+
+def _declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mf_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_all_declared_symbols():
+    from multifil_jax_amd import _lib
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert set(declared) == set(_lib.EXPORTED)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.mf_abi_version() == _lib.MF_ABI_VERSION
+
+
+def test_host_entry_points():
+    from multifil_jax_amd import _lib
+    from multifil_jax_amd.engine import state_layout
+    from multifil_jax_amd.topology import SarcTopology
+    lib = _lib.load()
+    t = SarcTopology.create(2, 2)
+    assert lib.mf_state_bytes(t.n_thick, t.n_crowns, t.n_thin, t.n_sites, t.n_xb_per_crown) \
+        == state_layout(t)["bytes"]
+    assert lib.mf_table_floats(200) == 16 + 27 * 16 + 400 * 48 + 400 * 4
+    # errors are reported, not raised across the boundary
+    L = _lib.MfLaunch()
+    L.block = 100
+    assert lib.mf_run(ctypes.byref(L), None) != 0
+    assert b"block" in lib.mf_last_error()
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_struct_layouts_match_c(tmp_path):
+    from multifil_jax_amd import _lib
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "multifil_amd.h"\n'
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mf_lattice), '
+                   'sizeof(mf_instance), sizeof(mf_solver_cfg), sizeof(mf_launch), '
+                   'offsetof(mf_launch, cfg), offsetof(mf_launch, lattices_host));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(_lib.MfLattice), ctypes.sizeof(_lib.MfInstance),
+            ctypes.sizeof(_lib.MfSolverCfg), ctypes.sizeof(_lib.MfLaunch),
+            _lib.MfLaunch.cfg.offset, _lib.MfLaunch.lattices_host.offset]
+    assert got == want
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from multifil_jax_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.load()

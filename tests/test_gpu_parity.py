@@ -1,0 +1,220 @@
+"""GPU parity of the gfx950 step engine against the FP64 oracle.
+
+Tolerances (stated per check):
+* transition tables (built in FP64 on the device, stored as FP32 cumulative
+  rows): within one float32 ulp of the FP64 oracle's cumulative rows;
+* discrete state (TM, XB, bindings): identical given identical uniforms;
+  aggregate agreement >= 99.99 %;
+* refined solver: max |dx| <= 1e-6 nm and M-line force within 1e-7 relative
+  of the exact FP64 equilibrium; reference solver: residual within the
+  reference's floor max(solver_tol, thick_k 1e-4, 0.25) plus the FP32
+  residual noise, positions within 0.01 nm of the exact equilibrium.
+"""
+import numpy as np
+import pytest
+
+from oracle import mf_oracle as mo
+from oracle.philox import uniforms, PURPOSE_TM, PURPOSE_XB
+
+from gpu_helpers import (TAB_TM, TAB_XB, tab_abs, oracle_state_to_device, make_batch,
+                         drivers_const, oracle_trajectory)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    return torch
+
+
+def _cfg(mode):
+    from multifil_jax_amd.engine import SolverConfig
+    return SolverConfig(mode=mode)
+
+
+def test_transition_tables(torch_cuda, topo22, skeletal, pdict):
+    _, d = skeletal
+    conds = [(4.0, 12.0), (4.5, 14.0), (5.5, 14.0), (7.0, 16.0)]
+    B = len(conds)
+    drv = np.zeros((B, 1, 3), np.float32)
+    for b, (pc, ls) in enumerate(conds):
+        drv[b, 0] = (900.0, pc, ls)
+    bt = make_batch([topo22], [0] * B, d.rows(B), drv, np.arange(B, dtype=np.uint64))
+    bt.run(0, cfg=_cfg("reference"))
+    tab = bt.d_tables.cpu().numpy()
+    nb = topo22.n_xb_bins
+    for b, (pc, ls) in enumerate(conds):
+        base = int(bt.table_off[b])
+        T = mo.make_tables(pdict, pc, ls, 1.0, topo22.xb_bin_centers)
+        tm = tab[base + TAB_TM: base + TAB_TM + 27 * 16].reshape(27, 4, 4)
+        assert np.max(np.abs(tm - np.cumsum(T.P_tm, axis=2))) < 2e-6
+        xb = tab[base + TAB_XB: base + TAB_XB + 2 * nb * 48].reshape(2 * nb, 6, 8)[:, :, :6]
+        want = np.cumsum(T.P_xb, axis=2)
+        assert np.max(np.abs(xb - want)) < 2e-6
+        # the stored rows are the FP64 reference rows rounded to FP32: within
+        # one float32 ulp (so every entry is within 1e-5 relative of the FP64
+        # oracle wherever the cumulative value is >= 0.012)
+        ulp = np.spacing(want.astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(xb - want) <= ulp * 1.0 + 1e-12)
+        tmw = np.cumsum(T.P_tm, axis=2)
+        assert np.all(np.abs(tm - tmw) <= np.spacing(tmw.astype(np.float32)).astype(np.float64) + 1e-12)
+        ab = tab[base + tab_abs(nb): base + tab_abs(nb) + 2 * nb * 4].reshape(2 * nb, 4)
+        for col, (i, j) in enumerate([(3, 4), (2, 0), (3, 0)]):
+            assert np.max(np.abs(ab[:, col] - T.P_abs[:, i, j])) < 2e-6
+
+
+def _teacher_forced(torch, topo, d, p, mode, n_starts, pCa, z, seed, steps_between=4):
+    """Run the oracle; from each visited state run one device step with the
+    same uniforms; return per-start comparisons."""
+    states, tab = oracle_trajectory(topo, p, z, pCa, n_starts * steps_between, seed,
+                                    mode="exact" if mode == "refined" else "reference")
+    S, X = topo.n_thin * topo.n_sites, topo.total_xbs
+    cfg = mo.SolverConfig(mode="exact" if mode == "refined" else "reference")
+    pre = mo.preconditioner(topo, p)
+    starts = list(range(steps_between, n_starts * steps_between + 1, steps_between))
+    B = len(starts)
+    drv = drivers_const(B, 1, z, pCa)
+    bt = make_batch([topo], [0] * B, d.rows(B), drv, np.arange(B, dtype=np.uint64))
+    U = np.zeros((B, 1, S + X), np.float32)
+    expect = []
+    for b, k in enumerate(starts):
+        st = states[k]
+        bt.set_state(b, oracle_state_to_device(st, topo))
+        u_tm = uniforms(1000 + k, 0, PURPOSE_TM, S)
+        u_xb = uniforms(1000 + k, 0, PURPOSE_XB, X)
+        U[b, 0, :S], U[b, 0, S:] = u_tm, u_xb
+        new, met = mo.step(st, topo, p, tab, z, pCa, 14.0, 0.0, u_tm, u_xb, cfg, pre)
+        expect.append((new, met))
+    bt.run(1, cfg=_cfg(mode), uniforms=torch.from_numpy(U).cuda())
+    got = [bt.get_state(b) for b in range(B)]
+    return expect, got, bt.metrics()[:, 0], bt.force()[:, 0]
+
+
+@pytest.mark.parametrize("mode", ["refined", "reference"])
+@pytest.mark.parametrize("lat", [2, 4])
+def test_teacher_forced_step(torch_cuda, skeletal, pdict, mode, lat):
+    from multifil_jax_amd.topology import SarcTopology
+    s, d = skeletal
+    topo = SarcTopology.create(lat, lat, s, d)
+    n_starts = 6 if lat == 2 else 3
+    expect, got, met, frc = _teacher_forced(torch_cuda, topo, d, pdict, mode, n_starts,
+                                            pCa=4.5, z=1000.0, seed=11)
+    agree = total = 0
+    tol = mo.solver_tolerance(pdict)
+    for (new, om), g, m, f in zip(expect, got, met, frc):
+        agree += np.sum(g["tm"] == new.tm.ravel()) + np.sum(g["xb_state"] == new.xb_state)
+        total += new.tm.size + new.xb_state.size
+        if np.array_equal(g["xb_state"], new.xb_state) and np.array_equal(g["xb_bound"], new.xb_bound):
+            assert np.array_equal(g["thin_bound"], new.thin_bound.ravel())
+            if mode == "refined":
+                assert np.max(np.abs(g["pos"] - new.x())) < 1e-6
+                assert abs(f - om["axial_force"]) <= 1e-7 * abs(om["axial_force"])
+                assert m[1] <= 1e-5
+            else:
+                assert m[1] <= tol or m[51] == 4
+                assert np.max(np.abs(g["pos"] - new.x())) < 1e-2
+    assert agree / total >= 0.9999, f"state agreement {agree}/{total}"
+
+
+def test_free_run_force_error(torch_cuda, topo22, skeletal, pdict):
+    """Refined device run vs FP64 oracle with identical Philox draws: the
+    M-line force stays within 1e-5 relative at every step."""
+    _, d = skeletal
+    T, B = 30, 3
+    z = np.array([900.0, 1000.0, 1100.0], np.float32)
+    pc = np.array([4.5, 5.0, 6.0], np.float32)
+    drv = drivers_const(B, T, z, pc)
+    bt = make_batch([topo22], [0] * B, d.rows(B), drv, np.arange(B, dtype=np.uint64))
+    bt.run(T, cfg=_cfg("refined"))
+    fg = bt.force()
+    mg = bt.metrics()
+    for b in range(B):
+        _, out = mo.run(topo22, pdict, drv[b, :, 0], drv[b, :, 1], drv[b, :, 2], 1.0, b,
+                        mo.SolverConfig(mode="exact"))
+        same = np.all(mg[b, :, 6:12] == out[:, 6:12], axis=1)   # XB state counts agree
+        assert same[:10].all()
+        rel = np.abs(fg[b] - out[:, 0]) / np.abs(out[:, 0])
+        assert np.max(rel[same]) < 1e-5
+
+
+def test_metrics_match_oracle(torch_cuda, topo22, skeletal, pdict):
+    _, d = skeletal
+    expect, got, met, frc = _teacher_forced(torch_cuda, topo22, d, pdict, "refined", 4,
+                                            pCa=5.0, z=1050.0, seed=5, steps_between=5)
+    names = mo.METRIC_NAMES
+    for (new, om), m in zip(expect, met):
+        ref = np.array([om[k] for k in names])
+        for i, k in enumerate(names):
+            a, b = float(m[i]), float(ref[i])
+            if np.isnan(b):
+                assert np.isnan(a), k
+                continue
+            if k in ("solver_residual",):
+                assert a <= 1e-5
+            elif k == "newton_iters":
+                assert 0 <= a <= 8
+            else:
+                assert abs(a - b) <= 2e-5 * max(1.0, abs(b)) + 1e-5, (k, a, b)
+
+
+def test_batch_composition_and_launch_split_are_bit_exact(torch_cuda, topo22, skeletal):
+    """An instance's trajectory is independent of its batch mates and of how
+    the steps are split into launches (persistent vs one launch per step)."""
+    _, d = skeletal
+    T = 12
+    drv4 = drivers_const(4, T, np.array([900, 1000, 1100, 950], np.float32),
+                         np.array([4.5, 5.0, 6.0, 5.5], np.float32))
+    seeds = np.array([3, 4, 5, 6], np.uint64)
+    a = make_batch([topo22], [0] * 4, d.rows(4), drv4, seeds)
+    a.run(T, cfg=_cfg("refined"))
+    b = make_batch([topo22], [0], d.rows(1), drv4[1:2], seeds[1:2])
+    for t in range(T):
+        b.run(1, t0=t, cfg=_cfg("refined"))
+    assert np.array_equal(a.force()[1], b.force()[0])
+    assert np.array_equal(a.metrics()[1], b.metrics()[0])
+    sa, sb = a.get_state(1), b.get_state(0)
+    for k in sa:
+        assert np.array_equal(np.asarray(sa[k]), np.asarray(sb[k])), k
+
+
+def test_no_activation_at_pCa9(torch_cuda, topo22, skeletal):
+    _, d = skeletal
+    drv = drivers_const(2, 40, 1000.0, 9.0)
+    bt = make_batch([topo22], [0, 0], d.rows(2), drv, np.array([1, 2], np.uint64))
+    bt.run(40, cfg=_cfg("refined"))
+    m = bt.metrics()
+    assert np.all(m[:, :, 5] <= 2)      # essentially nothing binds without calcium
+    assert np.all(m[:, :, 1] <= 1e-5)
+
+
+def test_full_size_invariants(torch_cuda, topo22, skeletal):
+    """256-instance sweep, 60 steps: size-independent invariants."""
+    _, d = skeletal
+    B, T = 256, 60
+    rng = np.random.default_rng(20261203)
+    drv = drivers_const(B, T, rng.uniform(850, 1150, B).astype(np.float32),
+                        rng.uniform(4.0, 7.0, B).astype(np.float32))
+    bt = make_batch([topo22], [0] * B, d.rows(B), drv, np.arange(B, dtype=np.uint64))
+    bt.run(T, cfg=_cfg("refined"))
+    m = bt.metrics()
+    assert np.all(np.isfinite(m[:, :, :28]))
+    assert np.all(m[:, :, 1] <= 1e-5)                              # refined residual
+    assert np.allclose(m[:, :, 6:12].sum(axis=2), topo22.total_xbs)
+    assert np.allclose(m[:, :, 19:23].sum(axis=2), topo22.n_thin * topo22.n_sites)
+    assert np.allclose(m[:, :, 5], m[:, :, 7:10].sum(axis=2))
+    for b in range(0, B, 37):
+        st = bt.get_state(b)
+        bound = (st["xb_state"] >= 1) & (st["xb_state"] <= 3)
+        assert np.array_equal(bound, st["xb_bound"] >= 0)
+        owners = st["thin_bound"][st["thin_bound"] >= 0]
+        assert np.all(bound[owners])
+        # every owner is bound to the site that names it
+        sites = np.nonzero(st["thin_bound"] >= 0)[0]
+        thin = topo22.xb_to_thin_id[owners]
+        assert np.array_equal(thin * topo22.n_sites + st["xb_bound"][owners], sites)
+    # force trace: activated instances generate force above the resting level
+    hi = drv[:, 0, 1] < 5.0
+    assert np.mean(m[hi, -1, 0] - m[hi, 0, 0]) > 50.0
