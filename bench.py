@@ -122,6 +122,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--mode", default="refined", choices=["refined", "reference"])
     ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--n-cg", type=int, default=6, help="CG iterations per Newton step")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/oracle/cpu legs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -130,7 +131,7 @@ def main():
         sys.exit(2)
     import torch
     from multifil_jax_amd.engine import SolverConfig
-    cfg = SolverConfig(mode=args.mode)
+    cfg = SolverConfig(mode=args.mode, n_cg=args.n_cg)
     B, K, W = args.batch, args.steps, args.warmup
     bt, topo, d, drv = make_batch(B, W + K)
     bt.block = args.block
@@ -158,7 +159,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched sweep: 256 conditions on a 2x2 lattice, pCa U[4,7], "
                                "z_line U[850,1150] nm, dt 1 ms, seeds 0..255, fixed spacing 14 nm",
-                   "lattice": "2x2", "batch": B, "solver": args.mode, "block": bt.block},
+                   "lattice": "2x2", "batch": B, "solver": args.mode, "block": bt.block,
+                   "n_cg": args.n_cg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "host_s": t_host,

@@ -66,6 +66,10 @@ def load() -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    # PyTorch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7); loading torch first makes the engine bind to that
+    # copy, so device buffers and streams come from a single runtime.
+    import torch  # noqa: F401
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "

@@ -56,46 +56,57 @@ __device__ __forceinline__ float philox_uniform(uint64_t seed, int step, int pur
 }
 
 // ---------------------------------------------------------------------------
-// wave-level primitives (wave64)
+// wave-level primitives (wave64, DPP)
+//
+// Inclusive prefix: Hillis-Steele inside each 16-lane row with row_shr:1/2/4/8,
+// then row_bcast:15 (rows 1, 3 take lane 15 of the row below) and
+// row_bcast:31 (rows 2, 3 take lane 31).  Totals are read from lane 63.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
+struct OpMax { template <typename T> __device__ T operator()(T a, T b) const { return fmax(a, b); } };
+
+// inclusive prefix over the 64 lanes (lane 0 first)
+template <typename T, typename Op = OpAdd>
+__device__ __forceinline__ T wave_prefix(T v, int lane, Op op = Op()) {
+  const int rl = lane & 15;
+  T t;
+  t = dpp_mov<0x111>(v); if (rl >= 1) v = op(t, v);
+  t = dpp_mov<0x112>(v); if (rl >= 2) v = op(t, v);
+  t = dpp_mov<0x114>(v); if (rl >= 4) v = op(t, v);
+  t = dpp_mov<0x118>(v); if (rl >= 8) v = op(t, v);
+  t = dpp_mov<0x142>(v); if ((lane & 31) >= 16) v = op(t, v);
+  t = dpp_mov<0x143>(v); if (lane >= 32) v = op(t, v);
   return v;
 }
+// exclusive prefix (0 on lane 0), from the inclusive one shifted by a lane
 template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
-  return v;
+__device__ __forceinline__ T wave_prefix_excl(T incl, int lane) {
+  const T t = dpp_mov<0x138>(incl);  // wave_shr:1
+  return lane == 0 ? T(0) : t;
 }
+// wave-uniform reductions
 template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, WAVE));
-  return v;
-}
-// inclusive prefix (lane 0 first)
+__device__ __forceinline__ T wave_sum(T v, int lane) { return read_lane(wave_prefix(v, lane, OpAdd()), 63); }
 template <typename T>
-__device__ __forceinline__ T wave_prefix(T v, int lane) {
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const T t = __shfl_up(v, o, WAVE);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
-// inclusive suffix (lane 63 first)
-template <typename T>
-__device__ __forceinline__ T wave_suffix(T v, int lane) {
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const T t = __shfl_down(v, o, WAVE);
-    if (lane + o < WAVE) v += t;
-  }
-  return v;
-}
+__device__ __forceinline__ T wave_max(T v, int lane) { return read_lane(wave_prefix(v, lane, OpMax()), 63); }
 
 // ---------------------------------------------------------------------------
 // rate laws (rate_functions.py) and generators (transitions.py)

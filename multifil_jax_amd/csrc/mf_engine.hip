@@ -143,93 +143,110 @@ struct Slots {
   int nch;        // chains owned by this wave
 };
 
-// Workgroup-wide reductions through a ring of LDS scratch rows.
+// Integer metric counters (LDS, reset by thread 0 after each step's output).
+enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, NC = 16 };
+// Per-thick-filament metric values written by the owning lane.
+enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF };
+// Per-wave partial sums: pass 1 (node sums / maxima, XB expectations), pass 2.
+enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_ATPE, W_TEAR, W_SQK, W_SQN, NWP };
+
+// All LDS arrays of one workgroup.
+template <bool REF64>
+struct Shared {
+  float* x32;        // [N]   positions (reference mode)
+  double* x64;       // [N]   positions (refined mode)
+  float* pv;         // [N]   CG direction
+  float* fx;         // [X]   XB axial force (reference mode)
+  double* fx64;      // [X]   XB axial force (refined mode)
+  float* kx;         // [X]   XB axial stiffness df/dx
+  short* xsite;      // [X]   DOF index of the bound site, -1 unbound
+  short* xnext;      // [X]   next XB in the site's bound list
+  int* xkey;         // [X]   table row of the last kinetics step
+  unsigned char* xst;  // [X] XB state
+  int* shead;        // [S]   first XB bound to the site, -1 none
+  int* tbound;       // [S]   thin.bound_to
+  unsigned char* tm; // [2][S] TM state (double buffered)
+  short* fsites;     // [Tn*nf*W] candidate sites per (thin, face)
+  short* fcount;     // [Tn*nf]
+  short* tprev;      // [S] same-chain neighbours
+  short* tnext;      // [S]
+  float* tmcum;      // [27*16]
+  float* edges;      // [nb+1]
+  float* prm;        // [MF_N_PARAMS]
+  float* red;        // [4][NW] scalar reduction ring
+  double* wpart;     // [NW][NWP] metric wave partials
+  double* fil;       // [Tk][NF] metric per-filament values
+  int* cnt;          // [NC] metric counters
+};
+
+template <int BS, bool REF64>
+__host__ __device__ inline int64_t lds_layout(const Dims& d, char* smem, Shared<REF64>* sh) {
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { char* p = smem ? smem + o : nullptr; o = (o + bytes + 15) & ~int64_t(15); return p; };
+  constexpr int NW = BS / WAVE;
+  Shared<REF64> t;
+  t.x32 = nullptr; t.x64 = nullptr; t.fx = nullptr; t.fx64 = nullptr;
+  if (REF64) t.x64 = (double*)take(8LL * d.N); else t.x32 = (float*)take(4LL * d.N);
+  t.pv = (float*)take(4LL * d.N);
+  if (REF64) t.fx64 = (double*)take(8LL * d.X); else t.fx = (float*)take(4LL * d.X);
+  t.kx = (float*)take(4LL * d.X);
+  t.xsite = (short*)take(2LL * d.X);
+  t.xnext = (short*)take(2LL * d.X);
+  t.xkey = (int*)take(4LL * d.X);
+  t.xst = (unsigned char*)take(d.X);
+  t.shead = (int*)take(4LL * d.S);
+  t.tbound = (int*)take(4LL * d.S);
+  t.tm = (unsigned char*)take(2LL * d.S);
+  t.fsites = (short*)take(2LL * d.Tn * d.nf * d.W);
+  t.fcount = (short*)take(2LL * d.Tn * d.nf);
+  t.tprev = (short*)take(2LL * d.S);
+  t.tnext = (short*)take(2LL * d.S);
+  t.tmcum = (float*)take(4 * 27 * 16);
+  t.edges = (float*)take(4LL * (d.nb + 1));
+  t.prm = (float*)take(4 * MF_N_PARAMS);
+  t.red = (float*)take(4 * 4 * NW);
+  t.wpart = (double*)take(8LL * NW * NWP);
+  t.fil = (double*)take(8LL * d.Tk * NF);
+  t.cnt = (int*)take(4 * NC);
+  if (sh) *sh = t;
+  return o;
+}
+template <int BS, bool REF64>
+int64_t lds_bytes(const Dims& d) { return lds_layout<BS, REF64>(d, nullptr, nullptr); }
+
+// Workgroup scalar reductions through a ring of LDS rows (one barrier each).
 template <int BS>
 struct Red {
   static constexpr int NW = BS / WAVE;
-  static constexpr int RK = 8;          // values per reduction row
-  double* buf;                          // [4][RK][NW]
+  float* buf;  // [4][NW]
   int ring;
-  double* big;                          // [MQ * (NW + 1)] metric reductions
-  __device__ double* row() { double* p = buf + (ring & 3) * RK * NW; ++ring; return p; }
+  __device__ float* row() { float* p = buf + (ring & 3) * NW; ++ring; return p; }
 };
-
-template <int K, int BS>
-__device__ __forceinline__ void wg_sum(Red<BS>& rd, double (&v)[K], int lane, int wave) {
-  static_assert(K <= Red<BS>::RK, "too many values");
-  double* r = rd.row();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double s = wave_sum(v[k]);
-    if (lane == 0) r[k * Red<BS>::NW + wave] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < Red<BS>::NW; ++w) s += r[k * Red<BS>::NW + w];
-    v[k] = s;
-  }
-}
 template <int BS>
-__device__ __forceinline__ float wg_sum1f(Red<BS>& rd, float v, int lane, int wave) {
-  double* r = rd.row();
-  const float s = wave_sum(v);
-  if (lane == 0) r[wave] = (double)s;
+__device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave) {
+  float* r = rd.row();
+  const float s = wave_sum(v, lane);
+  if (lane == 0) r[wave] = s;
   __syncthreads();
   float t = 0.0f;
 #pragma unroll
-  for (int w = 0; w < Red<BS>::NW; ++w) t += (float)r[w];
+  for (int w = 0; w < Red<BS>::NW; ++w) t += r[w];
   return t;
 }
 template <int BS>
-__device__ __forceinline__ double wg_max1(Red<BS>& rd, double v, int lane, int wave) {
-  double* r = rd.row();
-  const double s = wave_max(v);
+__device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave) {
+  float* r = rd.row();
+  const float s = wave_max(v, lane);
   if (lane == 0) r[wave] = s;
   __syncthreads();
-  double t = r[0];
+  float t = r[0];
 #pragma unroll
-  for (int w = 1; w < Red<BS>::NW; ++w) t = fmax(t, r[w]);
+  for (int w = 1; w < Red<BS>::NW; ++w) t = fmaxf(t, r[w]);
   return t;
 }
 
-// Metric partial sums (indices into one reduction row).  Entries from
-// Q_MAXK on are maxima (minima are reduced as maxima of the negation).
-enum MetricSlot {
-  Q_FORCE = 0, Q_SUMK, Q_SUMN, Q_E1, Q_DE1, Q_TIT, Q_DTIT, Q_WORKDX,
-  Q_XB0, Q_TM0 = Q_XB0 + 6, Q_OV = Q_TM0 + 4, Q_OV2, Q_OV3, Q_ATPC, Q_NEWLY, Q_ATPE, Q_TEAR,
-  Q_MAXK, Q_MAXN, Q_NMINK, Q_NMINN, MQ
-};
-
-// Two-level workgroup reduction of K values: entries [0, NSUM) summed, the
-// rest max-reduced.  Wave partials go to LDS, thread k folds value k over the
-// waves, and every thread gets a pointer to the K results in LDS.
-template <int K, int NSUM, int BS>
-__device__ __forceinline__ double* wg_reduce_mixed(Red<BS>& rd, double (&v)[K], int tid, int lane, int wave) {
-  constexpr int NW = BS / WAVE;
-  static_assert(K <= 40, "reduction scratch too small");
-  double* part = rd.big;       // [K][NW]
-  double* res = rd.big + K * NW;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double r = k < NSUM ? wave_sum(v[k]) : wave_max(v[k]);
-    if (lane == 0) part[k * NW + wave] = r;
-  }
-  __syncthreads();
-  if (tid < K) {
-    double a = part[tid * NW];
-    for (int w = 1; w < NW; ++w) a = tid < NSUM ? a + part[tid * NW + w] : fmax(a, part[tid * NW + w]);
-    res[tid] = a;
-  }
-  __syncthreads();
-  return res;
-}
-
 // Scan of slot values along every chain of this wave.  dir 0: inclusive
-// prefix (node order), 1: inclusive suffix.  Per-chain direction chosen by
-// `thick_dir`/`thin_dir`.
+// prefix in node order, 1: inclusive suffix.  Direction per chain type.
 template <int MS>
 __device__ __forceinline__ void chain_scan(float (&v)[MS], const Slots<MS>& sl, int lane, int thick_dir,
                                            int thin_dir) {
@@ -243,9 +260,7 @@ __device__ __forceinline__ void chain_scan(float (&v)[MS], const Slots<MS>& sl, 
 #pragma unroll
       for (int s = 0; s < MS; ++s)
         if (sl.chk[s] == k) { acc += v[s]; v[s] = acc; }
-      const float incl = wave_prefix(acc, lane);
-      float ex = __shfl_up(incl, 1, WAVE);
-      if (lane == 0) ex = 0.0f;
+      const float ex = wave_prefix_excl(wave_prefix(acc, lane), lane);
 #pragma unroll
       for (int s = 0; s < MS; ++s)
         if (sl.chk[s] == k) v[s] += ex;
@@ -253,9 +268,8 @@ __device__ __forceinline__ void chain_scan(float (&v)[MS], const Slots<MS>& sl, 
 #pragma unroll
       for (int s = MS - 1; s >= 0; --s)
         if (sl.chk[s] == k) { acc += v[s]; v[s] = acc; }
-      const float incl = wave_suffix(acc, lane);
-      float ex = __shfl_down(incl, 1, WAVE);
-      if (lane == WAVE - 1) ex = 0.0f;
+      const float incl = wave_prefix(acc, lane);
+      const float ex = read_lane(incl, 63) - incl;  // sum over the lanes after this one
 #pragma unroll
       for (int s = 0; s < MS; ++s)
         if (sl.chk[s] == k) v[s] += ex;
@@ -277,67 +291,6 @@ __device__ __forceinline__ void precond(const float (&r)[MS], float (&z)[MS], co
   chain_scan<MS>(z, sl, lane, 0, 1);
 #pragma unroll
   for (int s = 0; s < MS; ++s) z[s] *= sl.thick[s] ? inv_kk : inv_kn;
-}
-
-// All LDS arrays of one workgroup.
-template <bool REF64>
-struct Shared {
-  float* x32;        // [N]   positions (reference mode)
-  double* x64;       // [N]   positions (refined mode)
-  float* pv;         // [N]   CG direction
-  float* fx;         // [X]   XB axial force (reference mode)
-  double* fx64;      // [X]   XB axial force (refined mode)
-  float* kx;         // [X]   XB axial stiffness df/dx
-  short* xsite;      // [X]   DOF index of the bound site, -1 unbound
-  short* xnext;      // [X]   next XB in the site's bound list
-  int* xkey;         // [X]   table row of the last kinetics step
-  unsigned char* xst;  // [X] XB state
-  int* shead;        // [S]   first XB bound to the site, -1 none
-  int* tbound;       // [S]   thin.bound_to
-  unsigned char* tm; // [2][S] TM state (double buffered)
-  float* tmcum;      // [27*16]
-  float* edges;      // [nb+1]
-  float* prm;        // [MF_N_PARAMS]
-  double* red;       // reduction ring
-};
-
-template <int BS, bool REF64>
-__host__ __device__ inline int64_t lds_bytes(const Dims& d) {
-  int64_t o = 0;
-  auto take = [&](int64_t bytes) { const int64_t at = o; o = (o + bytes + 15) & ~int64_t(15); return at; };
-  if (REF64) take(8LL * d.N); else take(4LL * d.N);
-  take(4LL * d.N);
-  if (REF64) take(8LL * d.X); else take(4LL * d.X);
-  take(4LL * d.X); take(2LL * d.X); take(2LL * d.X); take(4LL * d.X); take(d.X);
-  take(4LL * d.S); take(4LL * d.S); take(2LL * d.S);
-  take(4 * 27 * 16); take(4LL * (d.nb + 1)); take(4 * MF_N_PARAMS);
-  take(8LL * (4 * 8 * (BS / WAVE) + 40 * (BS / WAVE + 1)));
-  return o;
-}
-
-template <int BS, bool REF64>
-__device__ inline Shared<REF64> carve(char* smem, const Dims& d) {
-  Shared<REF64> sh;
-  int64_t o = 0;
-  auto take = [&](int64_t bytes) { char* p = smem + o; o = (o + bytes + 15) & ~int64_t(15); return p; };
-  if (REF64) { sh.x64 = (double*)take(8LL * d.N); sh.x32 = nullptr; }
-  else { sh.x32 = (float*)take(4LL * d.N); sh.x64 = nullptr; }
-  sh.pv = (float*)take(4LL * d.N);
-  if (REF64) { sh.fx64 = (double*)take(8LL * d.X); sh.fx = nullptr; }
-  else { sh.fx = (float*)take(4LL * d.X); sh.fx64 = nullptr; }
-  sh.kx = (float*)take(4LL * d.X);
-  sh.xsite = (short*)take(2LL * d.X);
-  sh.xnext = (short*)take(2LL * d.X);
-  sh.xkey = (int*)take(4LL * d.X);
-  sh.xst = (unsigned char*)take(d.X);
-  sh.shead = (int*)take(4LL * d.S);
-  sh.tbound = (int*)take(4LL * d.S);
-  sh.tm = (unsigned char*)take(2LL * d.S);
-  sh.tmcum = (float*)take(4 * 27 * 16);
-  sh.edges = (float*)take(4LL * (d.nb + 1));
-  sh.prm = (float*)take(4 * MF_N_PARAMS);
-  sh.red = (double*)take(8LL * (4 * 8 * (BS / WAVE) + 40 * (BS / WAVE + 1)));
-  return sh;
 }
 
 template <bool REF64>
@@ -395,9 +348,10 @@ __device__ __forceinline__ void titin(const float* prm, T tip, T z, T d, T& tt, 
 // then per-node passive + XB sums (node phase).  Returns max|F| over the
 // workgroup; f[] receives the lane's slot residuals, ktip the tip stiffness.
 template <int BS, int MS, bool REF64, typename T>
-__device__ __forceinline__ double residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
-                                  const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
-                                  T (&f)[MS], float (&ktip)[MS], Red<BS>& rd, int tid, int lane, int wave) {
+__device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                          const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
+                                          T (&f)[MS], float (&ktip)[MS], Red<BS>& rd, int tid, int lane,
+                                          int wave) {
   const float* prm = sh.prm;
   for (int j = tid; j < dm.X; j += BS) {
     const int site = sh.xsite[j];
@@ -414,7 +368,7 @@ __device__ __forceinline__ double residual(const Shared<REF64>& sh, const Dims& 
   }
   __syncthreads();
   const T kk = T(prm[P_THICK_K]), kn = T(prm[P_THIN_K]);
-  double mx = 0.0;
+  float mx = 0.0f;
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     f[s] = T(0);
@@ -452,9 +406,11 @@ __device__ __forceinline__ double residual(const Shared<REF64>& sh, const Dims& 
       net -= xs;
     }
     f[s] = net;
-    mx = fmax(mx, fabs((double)net));
+    // max|F| is taken in float: it is compared with the stopping tolerance
+    // and reported as a float32 metric.
+    mx = fmaxf(mx, fabsf((float)net));
   }
-  return wg_max1<BS>(rd, mx, lane, wave);
+  return wg_max<BS>(rd, mx, lane, wave);
 }
 
 // -J p for the slot nodes (p read from LDS).
@@ -491,10 +447,12 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
 }
 
 // Unrolled preconditioned CG on A = -J from x0 = 0 (solver.py:329-366).  The
-// final iteration's unused preconditioner apply and rz reduction are skipped.
+// first pass's beta is 0; the last pass skips the unused preconditioner
+// apply and rz reduction.
 template <int BS, int MS, bool REF64>
-__device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl, const float (&b)[MS],
-                           const float (&ktip)[MS], int n_cg, float (&x)[MS], Red<BS>& rd, int lane, int wave) {
+__device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                    const float (&b)[MS], const float (&ktip)[MS], int n_cg, float (&x)[MS],
+                                    Red<BS>& rd, int lane, int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
   const float ikk = 1.0f / kk, ikn = 1.0f / kn;
   float r[MS], z[MS], p[MS], Ap[MS];
@@ -502,12 +460,11 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
   for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; }
   float rz = 0.0f;
   for (int it = 0;; ++it) {
-    // z = M^-1 r; rz = r.z; p = z + beta p (beta = 0 on the first pass)
     precond<MS>(r, z, sl, lane, ikk, ikn);
     float loc = 0.0f;
 #pragma unroll
     for (int s = 0; s < MS; ++s) loc += r[s] * z[s];
-    const float rz_new = wg_sum1f<BS>(rd, loc, lane, wave);
+    const float rz_new = wg_sum<BS>(rd, loc, lane, wave);
     const float beta = it == 0 ? 0.0f : rz_new / (rz + 1e-30f);
     rz = rz_new;
 #pragma unroll
@@ -520,7 +477,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
     loc = 0.0f;
 #pragma unroll
     for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
-    const float pAp = wg_sum1f<BS>(rd, loc, lane, wave);
+    const float pAp = wg_sum<BS>(rd, loc, lane, wave);
     const float alpha = rz / (pAp + 1e-30f);
 #pragma unroll
     for (int s = 0; s < MS; ++s) { x[s] += alpha * p[s]; r[s] -= alpha * Ap[s]; }
@@ -532,9 +489,10 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
 // Reference mode: FP32 residual, stop at max|F| <= tol or n_newton.
 // Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or n_refine.
 template <int BS, int MS, bool REF64>
-__device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl, const float (&rA)[MS],
-                             const float (&rB)[MS], float zl, float d, const mf_solver_cfg& cfg, Red<BS>& rd,
-                             int tid, int lane, int wave, int& iters, double& resid) {
+__device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                      const float (&rA)[MS], const float (&rB)[MS], float zl, float d,
+                                      const mf_solver_cfg& cfg, Red<BS>& rd, int tid, int lane, int wave,
+                                      int& iters, float& resid) {
   using T = typename std::conditional<REF64, double, float>::type;
   T f[MS];
   float ktip[MS], dx[MS], fb[MS];
@@ -549,10 +507,10 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
     cap = cfg.n_newton;
   }
   int it = 0;
-  double mx;
+  float mx;
   for (;;) {
     mx = residual<BS, MS, REF64, T>(sh, dm, sl, rA, rB, T(zl), T(d), f, ktip, rd, tid, lane, wave);
-    if (!(mx > (double)tol && it < cap)) break;
+    if (!(mx > tol && it < cap)) break;
 #pragma unroll
     for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
     pcg<BS, MS, REF64>(sh, dm, sl, fb, ktip, cfg.n_cg, dx, rd, lane, wave);
@@ -570,20 +528,20 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
   resid = mx;
 }
 
-// Nearest reachable site of XB j (geometry.py:60-175): search from crown+13,
-// mask padding slots and sites at x <= 0, first minimum wins; returns the
-// site index along the thin filament and the float32 axial offset from the crown.
+// Nearest reachable site (geometry.py:60-175): search from crown+13, mask
+// padding slots and sites at x <= 0, first minimum wins; returns the site
+// index along the thin filament and the float32 axial offset from the crown.
 template <bool REF64>
-__device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, const int32_t* face_sites,
-                                             const int32_t* face_count, int slot, int thin, int crown,
-                                             int& nearest, float& xdist) {
+__device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, int slot, int thin,
+                                             int crown, int& nearest, float& xdist) {
   const float base = posf<REF64>(sh, crown);
   const float head = base + XB_REACH;
-  const int cnt = face_count[slot];
-  const int32_t* cs = face_sites + (int64_t)slot * dm.W;
+  const int cnt = sh.fcount[slot];
+  const short* cs = sh.fsites + slot * dm.W;
   const int tb = dm.nk + thin * dm.ns;
   float best = INFINITY;
   int bj = 0;
+#pragma unroll 6
   for (int j = 0; j < dm.W; ++j) {
     const float sp = posf<REF64>(sh, tb + cs[j]);
     const float dd = fabsf(sp - head);
@@ -607,6 +565,12 @@ __device__ __forceinline__ int bin_of(const float* edges, int nb, float x) {
   return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
 }
 
+// Add popcount(pred over the wave) to an LDS counter (one atomic per wave).
+__device__ __forceinline__ void wave_count(int* c, bool pred, int lane) {
+  const unsigned long long m = __ballot(pred);
+  if (lane == 0 && m) atomicAdd(c, (int)__popcll(m));
+}
+
 template <int BS, int MS, int MX, bool REF64>
 __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   extern __shared__ __align__(16) char smem[];
@@ -621,28 +585,41 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   dm.W = G.width; dm.nf = G.n_faces; dm.nb = G.n_bins;
   dm.nk = dm.Tk * dm.nc; dm.N = dm.nk + dm.Tn * dm.ns; dm.X = dm.nk * dm.nx; dm.S = dm.Tn * dm.ns;
   dm.nch = dm.Tk + dm.Tn;
-  Shared<REF64> sh = carve<BS, REF64>(smem, dm);
-  Red<BS> rd{sh.red, 0, sh.red + 4 * Red<BS>::RK * (BS / WAVE)};
+  Shared<REF64> sh;
+  lds_layout<BS, REF64>(dm, smem, &sh);
+  Red<BS> rd{sh.red, 0};
 
   const float* lf = L.lat_f32;
   const int32_t* li = L.lat_i32;
-  const int32_t* xb_thin = li + G.xb_thin;
-  const int32_t* xb_slot = li + G.xb_slot;
-  const int32_t* xb_valid = li + G.xb_valid;
-  const int32_t* face_sites = li + G.face_sites;
-  const int32_t* face_count = li + G.face_count;
-  const int32_t* tm_prev = li + G.tm_prev;
-  const int32_t* tm_next = li + G.tm_next;
   const float* prm_g = L.params + (int64_t)b * MF_N_PARAMS;
   const float* drv = L.drivers + (int64_t)b * L.t_stride * 3;
   float* tab = L.tables + I.table_off;
   const StateLayout SL = state_layout(dm.Tk, dm.nc, dm.Tn, dm.ns, dm.nx);
   uint8_t* stb = L.state + I.state_off;
+  const int S = dm.S, X = dm.X, nb = dm.nb;
 
   // ---- static per-workgroup data ------------------------------------------
   for (int i = tid; i < MF_N_PARAMS; i += BS) sh.prm[i] = prm_g[i];
-  for (int i = tid; i <= dm.nb; i += BS) sh.edges[i] = lf[G.bin_edges + i];
+  for (int i = tid; i <= nb; i += BS) sh.edges[i] = lf[G.bin_edges + i];
   for (int i = tid; i < 27 * 16; i += BS) sh.tmcum[i] = tab[TAB_TM + i];
+  for (int i = tid; i < dm.Tn * dm.nf * dm.W; i += BS) sh.fsites[i] = (short)li[G.face_sites + i];
+  for (int i = tid; i < dm.Tn * dm.nf; i += BS) sh.fcount[i] = (short)li[G.face_count + i];
+  for (int i = tid; i < S; i += BS) { sh.tprev[i] = (short)li[G.tm_prev + i]; sh.tnext[i] = (short)li[G.tm_next + i]; }
+  for (int i = tid; i < NC; i += BS) sh.cnt[i] = 0;
+
+  // this thread's XBs: target thin, (thin, face) slot, validity
+  int q_thin[MX], q_slot[MX];
+  bool q_valid[MX];
+#pragma unroll
+  for (int q = 0; q < MX; ++q) {
+    const int j = tid + q * BS;
+    q_thin[q] = 0; q_slot[q] = 0; q_valid[q] = false;
+    if (j < X) {
+      q_thin[q] = li[G.xb_thin + j];
+      q_slot[q] = li[G.xb_slot + j];
+      q_valid[q] = li[G.xb_valid + j] != 0;
+    }
+  }
 
   // chain-major slot map
   Slots<MS> sl;
@@ -670,7 +647,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     }
     sl.nch = k;
   }
-  // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1}
+  // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1};
+  // co: crown offset (thick) or cumulative thin rest (thin), the displacement origins
   float rA[MS], rB[MS], co[MS];
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
@@ -702,15 +680,15 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     const int32_t* xbb = (const int32_t*)(stb + SL.xb_bound);
     const int32_t* xbk = (const int32_t*)(stb + SL.xb_key);
     const uint8_t* xbs = stb + SL.xb_state;
-    for (int j = tid; j < dm.X; j += BS) {
+    for (int j = tid; j < X; j += BS) {
       const int bt = xbb[j];
-      sh.xsite[j] = (short)(bt >= 0 ? dm.nk + xb_thin[j] * dm.ns + bt : -1);
+      sh.xsite[j] = (short)(bt >= 0 ? dm.nk + li[G.xb_thin + j] * dm.ns + bt : -1);
       sh.xkey[j] = xbk[j];
       sh.xst[j] = xbs[j];
     }
     const int32_t* tbd = (const int32_t*)(stb + SL.thin_bound);
     const uint8_t* tms = stb + SL.tm;
-    for (int s = tid; s < dm.S; s += BS) { sh.tbound[s] = tbd[s]; sh.tm[s] = tms[s]; }
+    for (int s = tid; s < S; s += BS) { sh.tbound[s] = tbd[s]; sh.tm[s] = tms[s]; }
     ls_carry = *(const double*)(stb + SL.ls);
   } else {
     // realize_state (state.py:191-262) at z0 = drivers[t0]
@@ -719,15 +697,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       const float v = i < dm.nk ? lf[G.crown_offsets + i] : z0 - lf[G.binding_offsets + (i - dm.nk)];
       if constexpr (REF64) sh.x64[i] = (double)v; else sh.x32[i] = v;
     }
-    for (int j = tid; j < dm.X; j += BS) { sh.xsite[j] = -1; sh.xst[j] = 0; }
-    for (int s = tid; s < dm.S; s += BS) { sh.tbound[s] = -1; sh.tm[s] = 0; }
+    for (int j = tid; j < X; j += BS) { sh.xsite[j] = -1; sh.xst[j] = 0; }
+    for (int s = tid; s < S; s += BS) { sh.tbound[s] = -1; sh.tm[s] = 0; }
     ls_carry = (double)drv[(I.t0 + L.t_base) * 3 + 2];
   }
-  for (int s = tid; s < dm.S; s += BS) sh.shead[s] = -1;
+  for (int s = tid; s < S; s += BS) sh.shead[s] = -1;
   __syncthreads();
 
   int tmc = 0;  // which TM buffer is current
-  const int S = dm.S, X = dm.X, nb = dm.nb;
   const float* xbtab = tab + TAB_XB;
   const float* abstab = tab + tab_abs(nb);
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
@@ -738,15 +715,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     const int t = I.t0 + L.t_base + (kin ? step : 0);
     const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
     double xpre[MS];
-    double m_atp_c = 0, m_newly = 0, m_atp_exp = 0, m_tear = 0;
     float z_solve, d_solve;
     if (!kin) {
       // Nearest update, then a solve against the parameter row's z_line and
       // lattice_spacing (the reference's run() leaves those at their defaults).
-      for (int j = tid; j < dm.X; j += BS) {
+      for (int j = tid; j < X; j += BS) {
         int nr; float xd;
-        nearest_site<REF64>(sh, dm, face_sites, face_count, xb_slot[j], xb_thin[j], j / dm.nx, nr, xd);
-        sh.xkey[j] = bin_of(sh.edges, dm.nb, xd);  // tm == 0 everywhere: permissiveness 0
+        nearest_site<REF64>(sh, dm, li[G.xb_slot + j], li[G.xb_thin + j], j / dm.nx, nr, xd);
+        sh.xkey[j] = bin_of(sh.edges, nb, xd);  // tm == 0 everywhere: permissiveness 0
       }
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
@@ -754,125 +730,131 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
 #pragma unroll
       for (int s = 0; s < MS; ++s) xpre[s] = 0.0;
     } else {
-    const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
-    const float dkin = lsd;
-    z_solve = zl;
-    d_solve = dkin;
-    // thin.axial += dz (simulation.py:683-687)
-    if (dz != 0.0f) {
-      for (int i = dm.nk + tid; i < dm.N; i += BS) {
-        if constexpr (REF64) sh.x64[i] += (double)dz; else sh.x32[i] += dz;
+      const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
+      z_solve = zl;
+      d_solve = lsd;
+      // thin.axial += dz (simulation.py:683-687)
+      if (dz != 0.0f) {
+        for (int i = dm.nk + tid; i < dm.N; i += BS) {
+          if constexpr (REF64) sh.x64[i] += (double)dz; else sh.x32[i] += dz;
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int s = 0; s < MS; ++s) xpre[s] = sl.node[s] >= 0 ? posd<REF64>(sh, sl.node[s]) : 0.0;
+
+      // ---- K1: TM transitions (transitions.py:516-625) + nearest sites -------
+      const unsigned char* tmo = sh.tm + tmc * S;
+      unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
+      for (int s = tid; s < S; s += BS) {
+        const int row = (s / dm.ns) * dm.ns, i = s - row;
+        const int pv = sh.tprev[s], nv = sh.tnext[s];
+        const int sp_ = tmo[row + pv], sn_ = tmo[row + nv];
+        const bool pr = pv != i, nr = nv != i;
+        const int n2 = (pr && sp_ == 2) + (nr && sn_ == 2);
+        const int n3 = (pr && sp_ == 3) + (nr && sn_ == 3);
+        const int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
+        const int cfg = 9 * n2 + 3 * n3 + ncl;
+        const int st = tmo[s];
+        const float u = U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(I.seed, t, 0, s);
+        int ns;
+        if (st == 3 && sh.tbound[s] >= 0) {
+          ns = 3;  // locked open under a bound head: [0, 0, 0, 1]
+        } else {
+          const float4 c = *(const float4*)(sh.tmcum + cfg * 16 + st * 4);
+          ns = u < c.x ? 0 : u < c.y ? 1 : u < c.z ? 2 : u < c.w ? 3 : 0;
+        }
+        tmn[s] = (unsigned char)ns;
+        sh.shead[s] = -1;
+      }
+      int q_near[MX], q_bin[MX], q_old[MX], q_keyold[MX];
+      bool q_occ[MX];
+#pragma unroll
+      for (int q = 0; q < MX; ++q) {
+        const int j = tid + q * BS;
+        q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_keyold[q] = 0; q_occ[q] = false;
+        if (j < X) {
+          float xd;
+          nearest_site<REF64>(sh, dm, q_slot[q], q_thin[q], j / dm.nx, q_near[q], xd);
+          q_bin[q] = bin_of(sh.edges, nb, xd);
+          q_occ[q] = sh.tbound[q_thin[q] * dm.ns + q_near[q]] >= 0;
+          q_old[q] = sh.xst[j];
+          q_keyold[q] = sh.xkey[j];
+        }
       }
       __syncthreads();
-    }
-    // pre-solve values kept for the metrics
-#pragma unroll
-    for (int s = 0; s < MS; ++s) xpre[s] = sl.node[s] >= 0 ? posd<REF64>(sh, sl.node[s]) : 0.0;
+      tmc ^= 1;
+      const unsigned char* tmc_p = sh.tm + tmc * S;
 
-    // ---- K1: TM transitions (transitions.py:516-625) + nearest sites ----------
-    const unsigned char* tmo = sh.tm + tmc * S;
-    unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
-    for (int s = tid; s < S; s += BS) {
-      const int thin = s / dm.ns, i = s - thin * dm.ns;
-      const int pv = tm_prev[s], nv = tm_next[s];
-      const int sp_ = tmo[thin * dm.ns + pv], sn_ = tmo[thin * dm.ns + nv];
-      const bool pr = pv != i, nr = nv != i;
-      int n2 = (pr && sp_ == 2) + (nr && sn_ == 2);
-      int n3 = (pr && sp_ == 3) + (nr && sn_ == 3);
-      int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
-      const int cfg = 9 * n2 + 3 * n3 + ncl;
-      const int st = tmo[s];
-      const float u = U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(I.seed, t, 0, s);
-      int ns;
-      if (st == 3 && sh.tbound[s] >= 0) {
-        ns = u < 1.0f ? 3 : 0;  // locked: [0, 0, 0, 1]
-      } else {
-        const float* c = sh.tmcum + cfg * 16 + st * 4;
-        ns = u < c[0] ? 0 : u < c[1] ? 1 : u < c[2] ? 2 : u < c[3] ? 3 : 0;
-      }
-      tmn[s] = (unsigned char)ns;
-      sh.shead[s] = -1;
-    }
-    int q_near[MX], q_bin[MX], q_old[MX], q_keyold[MX];
-    bool q_occ[MX];
+      // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
+      float m_atpe = 0.0f, m_tear = 0.0f;
 #pragma unroll
-    for (int q = 0; q < MX; ++q) {
-      const int j = tid + q * BS;
-      q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_keyold[q] = 0; q_occ[q] = false;
-      if (j < X) {
-        float xd;
-        nearest_site<REF64>(sh, dm, face_sites, face_count, xb_slot[j], xb_thin[j], j / dm.nx, q_near[q], xd);
-        q_bin[q] = bin_of(sh.edges, nb, xd);
-        q_occ[q] = sh.tbound[xb_thin[j] * dm.ns + q_near[q]] >= 0;
-        q_old[q] = sh.xst[j];
-        q_keyold[q] = sh.xkey[j];
-      }
-    }
-    __syncthreads();
-    tmc ^= 1;
-    const unsigned char* tmc_p = sh.tm + tmc * S;
-
-    // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
+      for (int q = 0; q < MX; ++q) {
+        const int j = tid + q * BS;
+        const bool on = j < X;
+        int nw = 0, old = 0;
+        if (on) {
+          const int thin = q_thin[q];
+          const int nsite = q_near[q];
+          const bool perm = (tmc_p[thin * dm.ns + nsite] == 3) && q_valid[q];
+          const int key = (perm ? nb : 0) + q_bin[q];
+          old = q_old[q];
+          const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(I.seed, t, 1, j);
+          const float4 c0 = *(const float4*)(xbtab + (int64_t)key * 48 + old * 8);
+          const float2 c1 = *(const float2*)(xbtab + (int64_t)key * 48 + old * 8 + 4);
+          nw = u < c0.x ? 0 : u < c0.y ? 1 : u < c0.z ? 2 : u < c0.w ? 3 : u < c1.x ? 4 : u < c1.y ? 5 : 0;
+          const bool ob = old >= 1 && old <= 3, nbd = nw >= 1 && nw <= 3;
+          const bool binding = !ob && nbd, unbinding = ob && !nbd;
+          const bool can_bind = binding && perm && !q_occ[q];
+          int site = sh.xsite[j];
+          if (unbinding) {
+            if (site >= 0) sh.tbound[site - dm.nk] = -1;
+            site = -1;
+          }
+          if (can_bind) {
+            site = dm.nk + thin * dm.ns + nsite;
+            atomicMin((unsigned int*)&sh.tbound[site - dm.nk], (unsigned int)j);
+          }
+          if (binding && !can_bind) nw = 0;
+          sh.xsite[j] = (short)site;
+          sh.xst[j] = (unsigned char)nw;
+          sh.xkey[j] = key;
+          if (site >= 0) sh.xnext[j] = (short)atomicExch(&sh.shead[site - dm.nk], j);
+          // expected ATP / tear from the previous step's key (metrics_fn.py:297-335)
+          if (old == 2 || old == 3) {
+            const float4 a = *(const float4*)(abstab + (int64_t)q_keyold[q] * 4);
+            if (old == 3) m_atpe += a.x;
+            m_tear += old == 3 ? a.z : a.y;
+          }
+        }
+        // state counts of the new XB states and per-step transition counts
 #pragma unroll
-    for (int q = 0; q < MX; ++q) {
-      const int j = tid + q * BS;
-      if (j >= X) continue;
-      const int thin = xb_thin[j];
-      const int nsite = q_near[q];
-      const bool perm = (tmc_p[thin * dm.ns + nsite] == 3) && (xb_valid[j] != 0);
-      const int key = (perm ? nb : 0) + q_bin[q];
-      const int old = q_old[q];
-      const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(I.seed, t, 1, j);
-      const float4 c0 = *(const float4*)(xbtab + (int64_t)key * 48 + old * 8);
-      const float2 c1 = *(const float2*)(xbtab + (int64_t)key * 48 + old * 8 + 4);
-      int nw = u < c0.x ? 0 : u < c0.y ? 1 : u < c0.z ? 2 : u < c0.w ? 3 : u < c1.x ? 4 : u < c1.y ? 5 : 0;
-      const bool ob = old >= 1 && old <= 3, nbd = nw >= 1 && nw <= 3;
-      const bool binding = !ob && nbd, unbinding = ob && !nbd;
-      const bool can_bind = binding && perm && !q_occ[q];
-      int site = sh.xsite[j];
-      if (unbinding) {
-        if (site >= 0) sh.tbound[site - dm.nk] = -1;
-        site = -1;
+        for (int k = 0; k < 6; ++k) wave_count(&sh.cnt[C_XB0 + k], on && nw == k, lane);
+        wave_count(&sh.cnt[C_ATPC], on && old == 3 && (nw == 4 || nw == 0), lane);
+        wave_count(&sh.cnt[C_NEWLY], on && old == 0 && nw == 1, lane);
       }
-      if (can_bind) {
-        site = dm.nk + thin * dm.ns + nsite;
-        atomicMin((unsigned int*)&sh.tbound[site - dm.nk], (unsigned int)j);
+      {
+        const float ae = wave_sum(m_atpe, lane), te = wave_sum(m_tear, lane);
+        if (lane == 0) { sh.wpart[wave * NWP + W_ATPE] = ae; sh.wpart[wave * NWP + W_TEAR] = te; }
       }
-      if (binding && !can_bind) nw = 0;
-      sh.xsite[j] = (short)site;
-      sh.xst[j] = (unsigned char)nw;
-      sh.xkey[j] = key;
-      if (site >= 0) sh.xnext[j] = (short)atomicExch(&sh.shead[site - dm.nk], j);
-      // metric terms on (old, new) and the previous step's key (metrics_fn.py:286-335)
-      m_atp_c += (old == 3 && (nw == 4 || nw == 0)) ? 1.0 : 0.0;
-      m_newly += (old == 0 && nw == 1) ? 1.0 : 0.0;
-      if (old == 2 || old == 3) {
-        const float4 a = *(const float4*)(abstab + (int64_t)q_keyold[q] * 4);
-        if (old == 3) m_atp_exp += (double)a.x;
-        m_tear += (double)(old == 3 ? a.z : a.y);
-      }
-    }
-    __syncthreads();
+      __syncthreads();
     }  // kinetics
 
     // ---- equilibrium solve -------------------------------------------------
     int iters;
-    double resid;
+    float resid;
     solve<BS, MS, REF64>(sh, dm, sl, rA, rB, z_solve, d_solve, L.cfg, rd, tid, lane, wave, iters, resid);
     if (!kin) continue;
     const double ls_new = (double)d_solve;
     ls_carry = ls_new;
-    const unsigned char* tmc_p = sh.tm + tmc * S;
 
     // ---- metrics (metrics_fn.py:154-417) -----------------------------------
     if (L.metrics != nullptr || L.force != nullptr) {
+      const unsigned char* tmc_p = sh.tm + tmc * S;
       const double kk = (double)sh.prm[P_THICK_K];
       const double ta = sh.prm[P_TITIN_A], tbp = sh.prm[P_TITIN_B], trest = sh.prm[P_TITIN_REST];
-      // pass 1: per-thread partial sums, one two-level reduction
-      double v[MQ];
-#pragma unroll
-      for (int k = 0; k < MQ; ++k) v[k] = 0.0;
-      v[Q_MAXK] = v[Q_MAXN] = v[Q_NMINK] = v[Q_NMINN] = -INFINITY;
+      // pass 1: node sums and extrema (wave partials), per-filament values
+      double sk = 0, sn = 0, wk = 0, mxk = -INFINITY, nmk = -INFINITY, mxn = -INFINITY, nmn = -INFINITY;
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
         const int n = sl.node[s];
@@ -880,95 +862,129 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const double x = posd<REF64>(sh, n);
         const double dd = x - (double)co[s];
         if (sl.thick[s]) {
-          v[Q_SUMK] += dd;
-          v[Q_MAXK] = fmax(v[Q_MAXK], dd);
-          v[Q_NMINK] = fmax(v[Q_NMINK], -dd);
-          v[Q_WORKDX] += x - xpre[s];
+          sk += dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
+          wk += x - xpre[s];
+          double* fv = sh.fil + sl.fil[s] * NF;
           if (sl.li[s] == 0) {
             const double e1 = 0.5 * kk * dd * dd;
             const double d0 = xpre[s] - (double)co[s];
-            v[Q_FORCE] += dd * kk;
-            v[Q_E1] += e1;
-            v[Q_DE1] += e1 - 0.5 * kk * d0 * d0;
+            fv[F_FORCE] = dd * kk;
+            fv[F_E1] = e1;
+            fv[F_DE1] = e1 - 0.5 * kk * d0 * d0;
           }
           if (sl.li[s] == dm.nc - 1) {
             const double an = (double)zl - x, ao = (double)zl - xpre[s];
             const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
             const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
             const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
-            v[Q_TIT] += en;
-            v[Q_DTIT] += en - eo;
+            fv[F_TIT] = en;
+            fv[F_DTIT] = en - eo;
           }
         } else {
-          v[Q_SUMN] += dd;
-          v[Q_MAXN] = fmax(v[Q_MAXN], dd);
-          v[Q_NMINN] = fmax(v[Q_NMINN], -dd);
+          sn += dd; mxn = fmax(mxn, dd); nmn = fmax(nmn, -dd);
         }
       }
-      for (int j = tid; j < X; j += BS) {
-        const int st = sh.xst[j];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) v[Q_XB0 + k] += st == k ? 1.0 : 0.0;
+      sk = wave_sum(sk, lane); sn = wave_sum(sn, lane); wk = wave_sum(wk, lane);
+      mxk = wave_max(mxk, lane); nmk = wave_max(nmk, lane);
+      mxn = wave_max(mxn, lane); nmn = wave_max(nmn, lane);
+      if (lane == 0) {
+        double* wp = sh.wpart + wave * NWP;
+        wp[W_SUMK] = sk; wp[W_SUMN] = sn; wp[W_WORK] = wk;
+        wp[W_MAXK] = mxk; wp[W_NMINK] = nmk; wp[W_MAXN] = mxn; wp[W_NMINN] = nmn;
       }
-      for (int s2 = tid; s2 < S; s2 += BS) {
-        const int st = tmc_p[s2];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[Q_TM0 + k] += st == k ? 1.0 : 0.0;
-        const float x = posf<REF64>(sh, dm.nk + s2);
-        if (x >= G.overlap_lo && x <= G.overlap_hi && x > 0.0f) {
-          v[Q_OV] += 1.0;
-          v[Q_OV2] += st == 2 ? 1.0 : 0.0;
-          v[Q_OV3] += st == 3 ? 1.0 : 0.0;
+      // TM state and overlap counts on the post-solve positions
+      for (int s2 = tid; s2 < S + (BS - S % BS) % BS; s2 += BS) {
+        const bool on = s2 < S;
+        const int st = on ? tmc_p[s2] : -1;
+        bool ov = false;
+        if (on) {
+          const float x = posf<REF64>(sh, dm.nk + s2);
+          ov = x >= G.overlap_lo && x <= G.overlap_hi && x > 0.0f;
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wave_count(&sh.cnt[C_TM0 + k], st == k, lane);
+        wave_count(&sh.cnt[C_OV], ov, lane);
+        wave_count(&sh.cnt[C_OV2], ov && st == 2, lane);
+        wave_count(&sh.cnt[C_OV3], ov && st == 3, lane);
       }
-      v[Q_ATPC] = m_atp_c; v[Q_NEWLY] = m_newly; v[Q_ATPE] = m_atp_exp; v[Q_TEAR] = m_tear;
-      double* fin = wg_reduce_mixed<MQ, Q_MAXK, BS>(rd, v, tid, lane, wave);
-      const double mean_k = fin[Q_SUMK] / dm.nk, mean_n = fin[Q_SUMN] / dm.S;
+      __syncthreads();
+      double sumk = 0, sumn = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) { sumk += sh.wpart[w * NWP + W_SUMK]; sumn += sh.wpart[w * NWP + W_SUMN]; }
+      const double mean_k = sumk / dm.nk, mean_n = sumn / S;
       // pass 2: population variances (jnp.std)
-      double w2[2] = {0.0, 0.0};
+      double qk = 0, qn = 0;
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
         const int n = sl.node[s];
         if (n < 0) continue;
         const double dd = posd<REF64>(sh, n) - (double)co[s];
-        if (sl.thick[s]) w2[0] += (dd - mean_k) * (dd - mean_k);
-        else w2[1] += (dd - mean_n) * (dd - mean_n);
+        if (sl.thick[s]) qk += (dd - mean_k) * (dd - mean_k);
+        else qn += (dd - mean_n) * (dd - mean_n);
       }
-      double* fin2 = wg_reduce_mixed<2, 2, BS>(rd, w2, tid, lane, wave);
+      qk = wave_sum(qk, lane);
+      qn = wave_sum(qn, lane);
+      if (lane == 0) { sh.wpart[wave * NWP + W_SQK] = qk; sh.wpart[wave * NWP + W_SQN] = qn; }
+      __syncthreads();
       if (tid == 0) {
+        double acc[NWP];
+#pragma unroll
+        for (int k = 0; k < NWP; ++k) acc[k] = 0.0;
+        acc[W_MAXK] = acc[W_NMINK] = acc[W_MAXN] = acc[W_NMINN] = -INFINITY;
+        // rolled loops: unrolled, every partial would be loaded at once
+#pragma unroll 1
+        for (int w = 0; w < NW; ++w) {
+          const double* wp = sh.wpart + w * NWP;
+#pragma unroll
+          for (int k = 0; k < NWP; ++k) {
+            const bool ismax = k == W_MAXK || k == W_NMINK || k == W_MAXN || k == W_NMINN;
+            acc[k] = ismax ? fmax(acc[k], wp[k]) : acc[k] + wp[k];
+          }
+        }
+        double fv[NF] = {0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int f = 0; f < dm.Tk; ++f)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) fv[k] += sh.fil[f * NF + k];
+        int c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) { c[k] = sh.cnt[k]; sh.cnt[k] = 0; }
         const double Xd = (double)X, Sd = (double)S;
-        const double force = fin[Q_FORCE];
-        const double nbound = fin[Q_XB0 + 1] + fin[Q_XB0 + 2] + fin[Q_XB0 + 3];
-        const double work = force * (fin[Q_WORKDX] / dm.nk);
-        const double ov = fin[Q_OV];
+        const double force = fv[F_FORCE];
+        const double nbound = (double)(c[C_XB0 + 1] + c[C_XB0 + 2] + c[C_XB0 + 3]);
+        const double work = force * (acc[W_WORK] / dm.nk);
+        const double ov = (double)c[C_OV];
         if (L.metrics) {
           float* m = L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS;
-          m[0] = (float)force; m[1] = (float)resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
+          m[0] = (float)force; m[1] = resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
           m[5] = (float)nbound;
-          for (int k = 0; k < 6; ++k) m[6 + k] = (float)fin[Q_XB0 + k];
+          for (int k = 0; k < 6; ++k) m[6 + k] = (float)c[C_XB0 + k];
           m[12] = (float)(nbound / Xd);
-          for (int k = 0; k < 6; ++k) m[13 + k] = (float)(fin[Q_XB0 + k] / Xd);
-          for (int k = 0; k < 4; ++k) m[19 + k] = (float)fin[Q_TM0 + k];
-          for (int k = 0; k < 4; ++k) m[23 + k] = (float)(fin[Q_TM0 + k] / Sd);
-          m[27] = (float)(fin[Q_TM0 + 3] / Sd);
-          m[28] = (float)(fin[Q_OV2] / ov);
-          m[29] = (float)(fin[Q_OV3] / ov);
-          m[30] = (float)((fin[Q_OV2] + fin[Q_OV3]) / ov);
+          for (int k = 0; k < 6; ++k) m[13 + k] = (float)(c[C_XB0 + k] / Xd);
+          for (int k = 0; k < 4; ++k) m[19 + k] = (float)c[C_TM0 + k];
+          for (int k = 0; k < 4; ++k) m[23 + k] = (float)(c[C_TM0 + k] / Sd);
+          m[27] = (float)(c[C_TM0 + 3] / Sd);
+          m[28] = (float)(c[C_OV2] / ov);
+          m[29] = (float)(c[C_OV3] / ov);
+          m[30] = (float)((c[C_OV2] + c[C_OV3]) / ov);
           m[31] = (float)ov;
-          m[32] = (float)fin[Q_ATPC]; m[33] = (float)fin[Q_NEWLY];
-          m[34] = (float)mean_k; m[35] = (float)fin[Q_MAXK]; m[36] = (float)(-fin[Q_NMINK]);
-          m[37] = (float)sqrt(fin2[0] / dm.nk);
-          m[38] = (float)mean_n; m[39] = (float)fin[Q_MAXN]; m[40] = (float)(-fin[Q_NMINN]);
-          m[41] = (float)sqrt(fin2[1] / dm.S);
-          m[42] = (float)(fin[Q_E1] / dm.Tk); m[43] = (float)(fin[Q_DE1] / dm.Tk);
-          m[44] = (float)(fin[Q_TIT] / dm.Tk); m[45] = (float)(fin[Q_DTIT] / dm.Tk);
+          m[32] = (float)c[C_ATPC]; m[33] = (float)c[C_NEWLY];
+          m[34] = (float)mean_k; m[35] = (float)acc[W_MAXK]; m[36] = (float)(-acc[W_NMINK]);
+          m[37] = (float)sqrt(acc[W_SQK] / dm.nk);
+          m[38] = (float)mean_n; m[39] = (float)acc[W_MAXN]; m[40] = (float)(-acc[W_NMINN]);
+          m[41] = (float)sqrt(acc[W_SQN] / S);
+          m[42] = (float)(fv[F_E1] / dm.Tk); m[43] = (float)(fv[F_DE1] / dm.Tk);
+          m[44] = (float)(fv[F_TIT] / dm.Tk); m[45] = (float)(fv[F_DTIT] / dm.Tk);
           m[46] = (float)work; m[47] = (float)(work / (double)dm.nk);
-          m[48] = (float)fin[Q_ATPE]; m[49] = (float)fin[Q_TEAR];
-          m[50] = (float)(fin[Q_ATPE] > 0.01 ? work / fin[Q_ATPE] : 0.0);
+          m[48] = (float)acc[W_ATPE]; m[49] = (float)acc[W_TEAR];
+          m[50] = (float)(acc[W_ATPE] > 0.01 ? work / acc[W_ATPE] : 0.0);
           m[51] = (float)iters;
         }
         if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
       }
+    } else if (tid == 0) {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) sh.cnt[k] = 0;
     }
   }
 
@@ -980,16 +996,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     int32_t* xbb = (int32_t*)(stb + SL.xb_bound);
     int32_t* xbk = (int32_t*)(stb + SL.xb_key);
     uint8_t* xbs = stb + SL.xb_state;
-    for (int j = tid; j < dm.X; j += BS) {
+    for (int j = tid; j < X; j += BS) {
       const int site = sh.xsite[j];
-      xbb[j] = site >= 0 ? site - dm.nk - xb_thin[j] * dm.ns : -1;
+      xbb[j] = site >= 0 ? site - dm.nk - li[G.xb_thin + j] * dm.ns : -1;
       xbk[j] = sh.xkey[j];
       xbs[j] = sh.xst[j];
     }
     int32_t* tbd = (int32_t*)(stb + SL.thin_bound);
     uint8_t* tms = stb + SL.tm;
-    const unsigned char* tcur = sh.tm + tmc * dm.S;
-    for (int s = tid; s < dm.S; s += BS) { tbd[s] = sh.tbound[s]; tms[s] = tcur[s]; }
+    const unsigned char* tcur = sh.tm + tmc * S;
+    for (int s = tid; s < S; s += BS) { tbd[s] = sh.tbound[s]; tms[s] = tcur[s]; }
     if (tid == 0) *(double*)(stb + SL.ls) = ls_carry;
   }
 }
