@@ -20,6 +20,11 @@
 
 using namespace mf;
 
+// Pointers into the workgroup's LDS carry the local address space so every
+// access compiles to ds_* instructions (a generic pointer would go through
+// flat addressing).
+#define LDS __attribute__((address_space(3)))
+
 namespace {
 
 thread_local std::string g_err;
@@ -143,6 +148,20 @@ struct Slots {
   int nch;        // chains owned by this wave
 };
 
+// XB partners of a lane's slot nodes, fixed during a solve: for a crown the
+// sites its bound heads hold, for a site the crowns of the heads bound to it
+// (list order).  Packed (partner DOF << 16 | XB index); -1 = none.  A site
+// with more than MAXP bound heads walks the LDS list instead (overflow).
+constexpr int MAXP = 4;
+template <int MS>
+struct Partners {
+  int pk[MS][MAXP];
+  float k[MS][MAXP];
+  bool overflow[MS];
+};
+__device__ __forceinline__ int p_node(int pk) { return pk >> 16; }
+__device__ __forceinline__ int p_xb(int pk) { return pk & 0xffff; }
+
 // Integer metric counters (LDS, reset by thread 0 after each step's output).
 enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, NC = 16 };
 // Per-thick-filament metric values written by the owning lane.
@@ -153,78 +172,114 @@ enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_
 // All LDS arrays of one workgroup.
 template <bool REF64>
 struct Shared {
-  float* x32;        // [N]   positions (reference mode)
-  double* x64;       // [N]   positions (refined mode)
-  float* pv;         // [N]   CG direction
-  float* fx;         // [X]   XB axial force (reference mode)
-  double* fx64;      // [X]   XB axial force (refined mode)
-  float* kx;         // [X]   XB axial stiffness df/dx
-  short* xsite;      // [X]   DOF index of the bound site, -1 unbound
-  short* xnext;      // [X]   next XB in the site's bound list
-  int* xkey;         // [X]   table row of the last kinetics step
-  unsigned char* xst;  // [X] XB state
-  int* shead;        // [S]   first XB bound to the site, -1 none
-  int* tbound;       // [S]   thin.bound_to
-  unsigned char* tm; // [2][S] TM state (double buffered)
-  short* fsites;     // [Tn*nf*W] candidate sites per (thin, face)
-  short* fcount;     // [Tn*nf]
-  short* tprev;      // [S] same-chain neighbours
-  short* tnext;      // [S]
-  float* tmcum;      // [27*16]
-  float* edges;      // [nb+1]
-  float* prm;        // [MF_N_PARAMS]
-  float* red;        // [4][NW] scalar reduction ring
-  double* wpart;     // [NW][NWP] metric wave partials
-  double* fil;       // [Tk][NF] metric per-filament values
-  int* cnt;          // [NC] metric counters
+  LDS float* x32;        // [N]   positions (reference mode)
+  LDS double* x64;       // [N]   positions (refined mode)
+  LDS float* pv;         // [N]   CG direction
+  LDS float* fx;         // [X]   XB axial force (reference mode)
+  LDS double* fx64;      // [X]   XB axial force (refined mode)
+  LDS float* kx;         // [X]   XB axial stiffness df/dx
+  LDS short* xsite;      // [X]   DOF index of the bound site, -1 unbound
+  LDS short* xnext;      // [X]   next XB in the site's bound list
+  LDS int* xkey;         // [X]   table row of the last kinetics step
+  LDS unsigned char* xst;  // [X] XB state
+  LDS int* shead;        // [S]   first XB bound to the site, -1 none
+  LDS int* tbound;       // [S]   thin.bound_to
+  LDS unsigned char* tm; // [2][S] TM state (double buffered)
+  LDS short* fsites;     // [Tn*nf*W] candidate sites per (thin, face)
+  LDS short* fcount;     // [Tn*nf]
+  LDS short* tprev;      // [S] same-chain neighbours
+  LDS short* tnext;      // [S]
+  LDS float* tmcum;      // [27*16]
+  LDS float* edges;      // [nb+1]
+  LDS float* prm;        // [MF_N_PARAMS]
+  LDS float* red;        // [4][NW] scalar reduction ring
+  LDS double* wpart;     // [NW][NWP] metric wave partials
+  LDS double* fil;       // [Tk][NF] metric per-filament values
+  LDS int* cnt;          // [NC] metric counters
 };
 
+// LDS carve-up.  One routine computes the byte offsets (host: the dynamic
+// LDS size; device: the typed pointers), so the two cannot disagree.
+struct LdsOffsets {
+  int64_t x, pv, fx, kx, xsite, xnext, xkey, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
+      edges, prm, red, wpart, fil, cnt, total;
+};
 template <int BS, bool REF64>
-__host__ __device__ inline int64_t lds_layout(const Dims& d, char* smem, Shared<REF64>* sh) {
-  int64_t o = 0;
-  auto take = [&](int64_t bytes) { char* p = smem ? smem + o : nullptr; o = (o + bytes + 15) & ~int64_t(15); return p; };
+__host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   constexpr int NW = BS / WAVE;
-  Shared<REF64> t;
-  t.x32 = nullptr; t.x64 = nullptr; t.fx = nullptr; t.fx64 = nullptr;
-  if (REF64) t.x64 = (double*)take(8LL * d.N); else t.x32 = (float*)take(4LL * d.N);
-  t.pv = (float*)take(4LL * d.N);
-  if (REF64) t.fx64 = (double*)take(8LL * d.X); else t.fx = (float*)take(4LL * d.X);
-  t.kx = (float*)take(4LL * d.X);
-  t.xsite = (short*)take(2LL * d.X);
-  t.xnext = (short*)take(2LL * d.X);
-  t.xkey = (int*)take(4LL * d.X);
-  t.xst = (unsigned char*)take(d.X);
-  t.shead = (int*)take(4LL * d.S);
-  t.tbound = (int*)take(4LL * d.S);
-  t.tm = (unsigned char*)take(2LL * d.S);
-  t.fsites = (short*)take(2LL * d.Tn * d.nf * d.W);
-  t.fcount = (short*)take(2LL * d.Tn * d.nf);
-  t.tprev = (short*)take(2LL * d.S);
-  t.tnext = (short*)take(2LL * d.S);
-  t.tmcum = (float*)take(4 * 27 * 16);
-  t.edges = (float*)take(4LL * (d.nb + 1));
-  t.prm = (float*)take(4 * MF_N_PARAMS);
-  t.red = (float*)take(4 * 4 * NW);
-  t.wpart = (double*)take(8LL * NW * NWP);
-  t.fil = (double*)take(8LL * d.Tk * NF);
-  t.cnt = (int*)take(4 * NC);
-  if (sh) *sh = t;
-  return o;
+  LdsOffsets L;
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { const int64_t at = o; o = (o + bytes + 15) & ~int64_t(15); return at; };
+  L.x = take((REF64 ? 8LL : 4LL) * d.N);
+  L.pv = take(4LL * d.N);
+  L.fx = take((REF64 ? 8LL : 4LL) * d.X);
+  L.kx = take(4LL * d.X);
+  L.xsite = take(2LL * d.X);
+  L.xnext = take(2LL * d.X);
+  L.xkey = take(4LL * d.X);
+  L.xst = take(d.X);
+  L.shead = take(4LL * d.S);
+  L.tbound = take(4LL * d.S);
+  L.tm = take(2LL * d.S);
+  L.fsites = take(2LL * d.Tn * d.nf * d.W);
+  L.fcount = take(2LL * d.Tn * d.nf);
+  L.tprev = take(2LL * d.S);
+  L.tnext = take(2LL * d.S);
+  L.tmcum = take(4 * 27 * 16);
+  L.edges = take(4LL * (d.nb + 1));
+  L.prm = take(4 * MF_N_PARAMS);
+  L.red = take(4 * 4 * NW);
+  L.wpart = take(8LL * NW * NWP);
+  L.fil = take(8LL * d.Tk * NF);
+  L.cnt = take(4 * NC);
+  L.total = o;
+  return L;
 }
 template <int BS, bool REF64>
-int64_t lds_bytes(const Dims& d) { return lds_layout<BS, REF64>(d, nullptr, nullptr); }
+int64_t lds_bytes(const Dims& d) { return lds_offsets<BS, REF64>(d).total; }
+
+template <int BS, bool REF64>
+__device__ __forceinline__ Shared<REF64> carve(LDS char* base, const Dims& d) {
+  const LdsOffsets o = lds_offsets<BS, REF64>(d);
+  Shared<REF64> t;
+  t.x32 = (LDS float*)(base + o.x);
+  t.x64 = (LDS double*)(base + o.x);
+  t.pv = (LDS float*)(base + o.pv);
+  t.fx = (LDS float*)(base + o.fx);
+  t.fx64 = (LDS double*)(base + o.fx);
+  t.kx = (LDS float*)(base + o.kx);
+  t.xsite = (LDS short*)(base + o.xsite);
+  t.xnext = (LDS short*)(base + o.xnext);
+  t.xkey = (LDS int*)(base + o.xkey);
+  t.xst = (LDS unsigned char*)(base + o.xst);
+  t.shead = (LDS int*)(base + o.shead);
+  t.tbound = (LDS int*)(base + o.tbound);
+  t.tm = (LDS unsigned char*)(base + o.tm);
+  t.fsites = (LDS short*)(base + o.fsites);
+  t.fcount = (LDS short*)(base + o.fcount);
+  t.tprev = (LDS short*)(base + o.tprev);
+  t.tnext = (LDS short*)(base + o.tnext);
+  t.tmcum = (LDS float*)(base + o.tmcum);
+  t.edges = (LDS float*)(base + o.edges);
+  t.prm = (LDS float*)(base + o.prm);
+  t.red = (LDS float*)(base + o.red);
+  t.wpart = (LDS double*)(base + o.wpart);
+  t.fil = (LDS double*)(base + o.fil);
+  t.cnt = (LDS int*)(base + o.cnt);
+  return t;
+}
 
 // Workgroup scalar reductions through a ring of LDS rows (one barrier each).
 template <int BS>
 struct Red {
   static constexpr int NW = BS / WAVE;
-  float* buf;  // [4][NW]
+  LDS float* buf;  // [4][NW]
   int ring;
-  __device__ float* row() { float* p = buf + (ring & 3) * NW; ++ring; return p; }
+  __device__ LDS float* row() { LDS float* p = buf + (ring & 3) * NW; ++ring; return p; }
 };
 template <int BS>
 __device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave) {
-  float* r = rd.row();
+  LDS float* r = rd.row();
   const float s = wave_sum(v, lane);
   if (lane == 0) r[wave] = s;
   __syncthreads();
@@ -235,7 +290,7 @@ __device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave
 }
 template <int BS>
 __device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave) {
-  float* r = rd.row();
+  LDS float* r = rd.row();
   const float s = wave_max(v, lane);
   if (lane == 0) r[wave] = s;
   __syncthreads();
@@ -304,7 +359,7 @@ __device__ __forceinline__ float posf(const Shared<REF64>& sh, int i) {
 
 // XB spring constants of a state (forces.py:378-394).
 struct Springs { float gk, gr, ck, cr; };
-__device__ __forceinline__ Springs springs(const float* prm, int st) {
+__device__ __forceinline__ Springs springs(const LDS float* prm, int st) {
   const bool strong = (st == 2) || (st == 3);
   Springs s;
   s.gk = strong ? prm[P_G_K_S] : prm[P_G_K_W];
@@ -316,11 +371,14 @@ __device__ __forceinline__ Springs springs(const float* prm, int st) {
 
 // Axial force of a bound XB at offset xd (forces.py:366-399) and its
 // derivative d f / d xd (what jax.jvp produces at solver.py:413).
+// In double the angle comes from a float atan2 (|error| ~1e-7 rad, i.e.
+// ~2e-7 pN per head, far below the refined solver's stopping tolerance); the
+// offsets, lengths and products stay in double.
 template <typename T>
 __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, float& kf) {
   const T r = sqrt(xd * xd + d * d);
   const T rs = r > T(1e-10) ? r : T(1e-10);
-  const T th = atan2(d, xd);
+  const T th = T(atan2f((float)d, (float)xd));
   const T cs = xd / rs, sn = d / rs;
   f = T(sp.gk) * (r - T(sp.gr)) * cs - (T(1) / rs) * T(sp.ck) * (th - T(sp.cr)) * sn;
   const T r2 = rs * rs;
@@ -331,7 +389,7 @@ __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, flo
 // Titin axial force on the tip crown and its derivative with respect to the
 // tip position (forces.py:136-157).
 template <typename T>
-__device__ __forceinline__ void titin(const float* prm, T tip, T z, T d, T& tt, float& ktt) {
+__device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt) {
   const T ad = z - tip;
   const T Lq = sqrt(ad * ad + d * d);
   const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
@@ -344,15 +402,49 @@ __device__ __forceinline__ void titin(const float* prm, T tip, T z, T d, T& tt, 
   ktt = (float)(-T(N_TITIN) * (Fp * ad * ad / (Lq * Lq) + F * d * d / (Lq * Lq * Lq)));
 }
 
+// Partners of the slot nodes for the current bindings (start of a solve).
+template <int MS, bool REF64>
+__device__ __forceinline__ void build_partners(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+                                               Partners<MS>& pt) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    pt.overflow[s] = false;
+#pragma unroll
+    for (int h = 0; h < MAXP; ++h) { pt.pk[s][h] = -1; pt.k[s][h] = 0.0f; }
+    const int n = sl.node[s];
+    if (n < 0) continue;
+    if (sl.thick[s]) {
+#pragma unroll
+      for (int h = 0; h < MAXP; ++h) {
+        if (h >= dm.nx) break;
+        const int j = n * dm.nx + h;
+        const int site = sh.xsite[j];
+        const int st = sh.xst[j];
+        if (site >= 0 && st >= 1 && st <= 3) pt.pk[s][h] = (site << 16) | j;
+      }
+    } else {
+      int j = sh.shead[n - dm.nk];
+#pragma unroll
+      for (int h = 0; h < MAXP; ++h) {
+        if (j < 0) break;
+        pt.pk[s][h] = ((j / dm.nx) << 16) | j;
+        j = sh.xnext[j];
+      }
+      pt.overflow[s] = j >= 0;
+    }
+  }
+}
+
 // Residual at the current positions: per-XB forces/stiffnesses (XB phase),
-// then per-node passive + XB sums (node phase).  Returns max|F| over the
-// workgroup; f[] receives the lane's slot residuals, ktip the tip stiffness.
+// then per-node passive + XB sums (node phase), which also refreshes the
+// partners' stiffnesses.  Returns max|F| over the workgroup; f[] receives the
+// lane's slot residuals, ktip the tip stiffness.
 template <int BS, int MS, bool REF64, typename T>
 __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
-                                          T (&f)[MS], float (&ktip)[MS], Red<BS>& rd, int tid, int lane,
-                                          int wave) {
-  const float* prm = sh.prm;
+                                          T (&f)[MS], float (&ktip)[MS], Partners<MS>& pt, Red<BS>& rd,
+                                          int tid, int lane, int wave) {
+  const LDS float* prm = sh.prm;
   for (int j = tid; j < dm.X; j += BS) {
     const int site = sh.xsite[j];
     T fj = T(0);
@@ -377,6 +469,15 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
     if (n < 0) continue;
     const int i = sl.li[s];
     const T x0 = T(posd<REF64>(sh, n));
+    T xs = T(0);
+#pragma unroll
+    for (int h = 0; h < MAXP; ++h) {
+      const int pk = pt.pk[s][h];
+      if (pk < 0) continue;
+      const int j = p_xb(pk);
+      if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+      pt.k[s][h] = sh.kx[j];
+    }
     T net;
     if (sl.thick[s]) {
       const T xm = i > 0 ? T(posd<REF64>(sh, n - 1)) : T(0);
@@ -388,20 +489,18 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
         titin<T>(prm, x0, zl, d, s_hi, ktip[s]);
       }
       net = s_hi - s_lo;
-      T xs = T(0);
-      for (int h = 0; h < dm.nx; ++h) {
-        const int j = n * dm.nx + h;
-        if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
-      }
       net += xs;
     } else {
       const T xp = i < dm.ns - 1 ? T(posd<REF64>(sh, n + 1)) : zl;
       const T s_hi = (xp - x0 - T(rA[s])) * kn;
       const T s_lo = i > 0 ? (x0 - T(posd<REF64>(sh, n - 1)) - T(rB[s])) * kn : T(0);
       net = s_hi - s_lo;
-      T xs = T(0);
-      for (int j = sh.shead[n - dm.nk]; j >= 0; j = sh.xnext[j]) {
-        if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+      if (pt.overflow[s]) {
+        int j = sh.shead[n - dm.nk];
+        for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
+        for (; j >= 0; j = sh.xnext[j]) {
+          if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+        }
       }
       net -= xs;
     }
@@ -416,7 +515,8 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
 // -J p for the slot nodes (p read from LDS).
 template <int MS, bool REF64>
 __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
-                                        const float (&ktip)[MS], float kk, float kn, float (&Ap)[MS]) {
+                                        const Partners<MS>& pt, const float (&ktip)[MS], float kk, float kn,
+                                        float (&Ap)[MS]) {
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     Ap[s] = 0.0f;
@@ -424,23 +524,28 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
     if (n < 0) continue;
     const int i = sl.li[s];
     const float p0 = sh.pv[n];
+    float q = 0.0f;  // sum over partners of k (p_partner - p0)
+#pragma unroll
+    for (int h = 0; h < MAXP; ++h) {
+      const int pk = pt.pk[s][h];
+      if (pk >= 0) q += pt.k[s][h] * (sh.pv[p_node(pk)] - p0);
+    }
     float net;
     if (sl.thick[s]) {
       const float pm = i > 0 ? sh.pv[n - 1] : 0.0f;
       const float lo = kk * (p0 - pm);
       const float hi = i < dm.nc - 1 ? kk * (sh.pv[n + 1] - p0) : ktip[s] * p0;
-      net = hi - lo;
-      for (int h = 0; h < dm.nx; ++h) {
-        const int j = n * dm.nx + h;
-        const int site = sh.xsite[j];
-        if (site >= 0) net += sh.kx[j] * (sh.pv[site] - p0);
-      }
+      net = hi - lo + q;
     } else {
       const float pp = i < dm.ns - 1 ? sh.pv[n + 1] : 0.0f;
       const float hi = kn * (pp - p0);
       const float lo = i > 0 ? kn * (p0 - sh.pv[n - 1]) : 0.0f;
-      net = hi - lo;
-      for (int j = sh.shead[n - dm.nk]; j >= 0; j = sh.xnext[j]) net -= sh.kx[j] * (p0 - sh.pv[j / dm.nx]);
+      if (pt.overflow[s]) {
+        int j = sh.shead[n - dm.nk];
+        for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
+        for (; j >= 0; j = sh.xnext[j]) q += sh.kx[j] * (sh.pv[j / dm.nx] - p0);
+      }
+      net = hi - lo + q;
     }
     Ap[s] = -net;
   }
@@ -451,8 +556,8 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
 // apply and rz reduction.
 template <int BS, int MS, bool REF64>
 __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
-                                    const float (&b)[MS], const float (&ktip)[MS], int n_cg, float (&x)[MS],
-                                    Red<BS>& rd, int lane, int wave) {
+                                    const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
+                                    int n_cg, float (&x)[MS], Red<BS>& rd, int lane, int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
   const float ikk = 1.0f / kk, ikn = 1.0f / kn;
   float r[MS], z[MS], p[MS], Ap[MS];
@@ -473,7 +578,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
       if (sl.node[s] >= 0) sh.pv[sl.node[s]] = p[s];
     }
     __syncthreads();
-    neg_jvp<MS, REF64>(sh, dm, sl, ktip, kk, kn, Ap);
+    neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, Ap);
     loc = 0.0f;
 #pragma unroll
     for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
@@ -506,14 +611,16 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
     tol = fmaxf(t0, fmaxf(sh.prm[P_THICK_K] * 1e-4f, 0.25f));
     cap = cfg.n_newton;
   }
+  Partners<MS> pt;
+  build_partners<MS, REF64>(sh, dm, sl, pt);
   int it = 0;
   float mx;
   for (;;) {
-    mx = residual<BS, MS, REF64, T>(sh, dm, sl, rA, rB, T(zl), T(d), f, ktip, rd, tid, lane, wave);
+    mx = residual<BS, MS, REF64, T>(sh, dm, sl, rA, rB, T(zl), T(d), f, ktip, pt, rd, tid, lane, wave);
     if (!(mx > tol && it < cap)) break;
 #pragma unroll
     for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
-    pcg<BS, MS, REF64>(sh, dm, sl, fb, ktip, cfg.n_cg, dx, rd, lane, wave);
+    pcg<BS, MS, REF64>(sh, dm, sl, pt, fb, ktip, cfg.n_cg, dx, rd, lane, wave);
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
       const int n = sl.node[s];
@@ -531,44 +638,59 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
 // Nearest reachable site (geometry.py:60-175): search from crown+13, mask
 // padding slots and sites at x <= 0, first minimum wins; returns the site
 // index along the thin filament and the float32 axial offset from the crown.
+// Widths up to 32 run fully unrolled so every candidate load is in flight at
+// once.
+constexpr int MAXW_UNROLLED = 32;
 template <bool REF64>
 __device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, int slot, int thin,
                                              int crown, int& nearest, float& xdist) {
   const float base = posf<REF64>(sh, crown);
   const float head = base + XB_REACH;
   const int cnt = sh.fcount[slot];
-  const short* cs = sh.fsites + slot * dm.W;
+  const LDS short* cs = sh.fsites + slot * dm.W;
   const int tb = dm.nk + thin * dm.ns;
   float best = INFINITY;
   int bj = 0;
-#pragma unroll 6
-  for (int j = 0; j < dm.W; ++j) {
-    const float sp = posf<REF64>(sh, tb + cs[j]);
-    const float dd = fabsf(sp - head);
-    const bool ok = (j < cnt) && (sp > 0.0f);
-    if (ok && dd < best) { best = dd; bj = j; }
+  if (dm.W <= MAXW_UNROLLED) {
+    float sp[MAXW_UNROLLED];
+#pragma unroll
+    for (int j = 0; j < MAXW_UNROLLED; ++j) sp[j] = j < dm.W ? posf<REF64>(sh, tb + cs[j]) : 0.0f;
+#pragma unroll
+    for (int j = 0; j < MAXW_UNROLLED; ++j) {
+      const float dd = fabsf(sp[j] - head);
+      const bool ok = (j < cnt) && (sp[j] > 0.0f);
+      if (ok && dd < best) { best = dd; bj = j; }
+    }
+  } else {
+    for (int j = 0; j < dm.W; ++j) {
+      const float spj = posf<REF64>(sh, tb + cs[j]);
+      const float dd = fabsf(spj - head);
+      const bool ok = (j < cnt) && (spj > 0.0f);
+      if (ok && dd < best) { best = dd; bj = j; }
+    }
   }
   nearest = cs[bj];
   xdist = posf<REF64>(sh, tb + nearest) - base;
 }
 
-__device__ __forceinline__ int bin_of(const float* edges, int nb, float x) {
-  // clip(digitize(x, edges) - 1, 0, nb - 1) (transitions.py:915-917): the
-  // number of edges <= x, minus one; NaN sorts after every edge.
+// clip(digitize(x, edges) - 1, 0, nb - 1) (transitions.py:915-917): the
+// number of edges <= x, minus one; NaN sorts after every edge.  The edges
+// are a float32 linspace, so the affine estimate is off by at most one bin;
+// two edge loads settle it.
+__device__ __forceinline__ int bin_of(const LDS float* edges, int nb, float inv_w, float x) {
   if (isnan(x)) return nb - 1;
-  int lo = 0, hi = nb + 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (edges[mid] <= x) lo = mid + 1; else hi = mid;
-  }
-  const int b = lo - 1;
-  return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+  const float t = fminf(fmaxf((x - edges[0]) * inv_w, 0.0f), (float)(nb - 1));
+  int i = (int)t;
+  const float a = edges[i], c = edges[i + 1];
+  if (x < a) --i;
+  else if (x >= c) ++i;
+  return i < 0 ? 0 : (i > nb - 1 ? nb - 1 : i);
 }
 
 // Add popcount(pred over the wave) to an LDS counter (one atomic per wave).
-__device__ __forceinline__ void wave_count(int* c, bool pred, int lane) {
+__device__ __forceinline__ void wave_count(LDS int* c, bool pred, int lane) {
   const unsigned long long m = __ballot(pred);
-  if (lane == 0 && m) atomicAdd(c, (int)__popcll(m));
+  if (lane == 0 && m) atomicAdd((int*)c, (int)__popcll(m));
 }
 
 template <int BS, int MS, int MX, bool REF64>
@@ -577,16 +699,17 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   constexpr int NW = BS / WAVE;
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
   const int b = blockIdx.x;
-  const mf_instance I = L.instances[b];
+  // descriptors are read in place (scalar loads); a by-value copy of these
+  // structs would live in scratch
+  const mf_instance& I = L.instances[b];
   if (I.n_steps <= 0 && !(I.flags & MF_INST_INIT)) return;
-  const mf_lattice G = L.lattices[I.lattice];
+  const mf_lattice& G = L.lattices[I.lattice];
   Dims dm;
   dm.Tk = G.n_thick; dm.nc = G.n_crowns; dm.Tn = G.n_thin; dm.ns = G.n_sites; dm.nx = G.n_xb_per_crown;
   dm.W = G.width; dm.nf = G.n_faces; dm.nb = G.n_bins;
   dm.nk = dm.Tk * dm.nc; dm.N = dm.nk + dm.Tn * dm.ns; dm.X = dm.nk * dm.nx; dm.S = dm.Tn * dm.ns;
   dm.nch = dm.Tk + dm.Tn;
-  Shared<REF64> sh;
-  lds_layout<BS, REF64>(dm, smem, &sh);
+  Shared<REF64> sh = carve<BS, REF64>((LDS char*)smem, dm);
   Red<BS> rd{sh.red, 0};
 
   const float* lf = L.lat_f32;
@@ -704,15 +827,20 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   for (int s = tid; s < S; s += BS) sh.shead[s] = -1;
   __syncthreads();
 
+  const uint64_t seed = I.seed;
+  const int n_steps = I.n_steps, t_first = I.t0 + L.t_base;
+  const float ov_lo = G.overlap_lo, ov_hi = G.overlap_hi;
   int tmc = 0;  // which TM buffer is current
+  const float inv_w = (float)nb / (lf[G.bin_edges + nb] - lf[G.bin_edges]);
+  const double inv_nk = 1.0 / (double)dm.nk, inv_S = 1.0 / (double)S;
   const float* xbtab = tab + TAB_XB;
   const float* abstab = tab + tab_abs(nb);
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
 
   // ---- time loop; step -1 is create_and_equilibrate (simulation.py:633-642)
-  for (int step = init ? -1 : 0; step < I.n_steps; ++step) {
+  for (int step = init ? -1 : 0; step < n_steps; ++step) {
     const bool kin = step >= 0;
-    const int t = I.t0 + L.t_base + (kin ? step : 0);
+    const int t = t_first + (kin ? step : 0);
     const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
     double xpre[MS];
     float z_solve, d_solve;
@@ -722,7 +850,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       for (int j = tid; j < X; j += BS) {
         int nr; float xd;
         nearest_site<REF64>(sh, dm, li[G.xb_slot + j], li[G.xb_thin + j], j / dm.nx, nr, xd);
-        sh.xkey[j] = bin_of(sh.edges, nb, xd);  // tm == 0 everywhere: permissiveness 0
+        sh.xkey[j] = bin_of(sh.edges, nb, inv_w, xd);  // tm == 0 everywhere: permissiveness 0
       }
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
@@ -744,8 +872,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       for (int s = 0; s < MS; ++s) xpre[s] = sl.node[s] >= 0 ? posd<REF64>(sh, sl.node[s]) : 0.0;
 
       // ---- K1: TM transitions (transitions.py:516-625) + nearest sites -------
-      const unsigned char* tmo = sh.tm + tmc * S;
-      unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
+      const LDS unsigned char* tmo = sh.tm + tmc * S;
+      LDS unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
       for (int s = tid; s < S; s += BS) {
         const int row = (s / dm.ns) * dm.ns, i = s - row;
         const int pv = sh.tprev[s], nv = sh.tnext[s];
@@ -756,35 +884,48 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
         const int cfg = 9 * n2 + 3 * n3 + ncl;
         const int st = tmo[s];
-        const float u = U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(I.seed, t, 0, s);
+        const float u = U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(seed, t, 0, s);
         int ns;
         if (st == 3 && sh.tbound[s] >= 0) {
           ns = 3;  // locked open under a bound head: [0, 0, 0, 1]
         } else {
-          const float4 c = *(const float4*)(sh.tmcum + cfg * 16 + st * 4);
-          ns = u < c.x ? 0 : u < c.y ? 1 : u < c.z ? 2 : u < c.w ? 3 : 0;
+          const LDS float* c = sh.tmcum + cfg * 16 + st * 4;
+          ns = u < c[0] ? 0 : u < c[1] ? 1 : u < c[2] ? 2 : u < c[3] ? 3 : 0;
         }
         tmn[s] = (unsigned char)ns;
         sh.shead[s] = -1;
       }
-      int q_near[MX], q_bin[MX], q_old[MX], q_keyold[MX];
+      // nearest sites, and the candidate table rows of both permissiveness
+      // levels fetched ahead of the barrier (K2 only picks one)
+      int q_near[MX], q_bin[MX], q_old[MX];
       bool q_occ[MX];
+      float4 q_r0[MX], q_r1[MX], q_ab[MX];
+      float2 q_s0[MX], q_s1[MX];
 #pragma unroll
       for (int q = 0; q < MX; ++q) {
         const int j = tid + q * BS;
-        q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_keyold[q] = 0; q_occ[q] = false;
+        q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_occ[q] = false;
+        q_r0[q] = q_r1[q] = q_ab[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        q_s0[q] = q_s1[q] = make_float2(0.f, 0.f);
         if (j < X) {
+          const int old = sh.xst[j];
+          const int keyold = sh.xkey[j];
           float xd;
           nearest_site<REF64>(sh, dm, q_slot[q], q_thin[q], j / dm.nx, q_near[q], xd);
-          q_bin[q] = bin_of(sh.edges, nb, xd);
+          const int bn = bin_of(sh.edges, nb, inv_w, xd);
+          const float* r0 = xbtab + (int64_t)bn * 48 + old * 8;
+          const float* r1 = xbtab + (int64_t)(nb + bn) * 48 + old * 8;
+          q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
+          q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
+          if (old == 2 || old == 3) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
+          q_bin[q] = bn;
           q_occ[q] = sh.tbound[q_thin[q] * dm.ns + q_near[q]] >= 0;
-          q_old[q] = sh.xst[j];
-          q_keyold[q] = sh.xkey[j];
+          q_old[q] = old;
         }
       }
       __syncthreads();
       tmc ^= 1;
-      const unsigned char* tmc_p = sh.tm + tmc * S;
+      const LDS unsigned char* tmc_p = sh.tm + tmc * S;
 
       // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
       float m_atpe = 0.0f, m_tear = 0.0f;
@@ -799,9 +940,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const bool perm = (tmc_p[thin * dm.ns + nsite] == 3) && q_valid[q];
           const int key = (perm ? nb : 0) + q_bin[q];
           old = q_old[q];
-          const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(I.seed, t, 1, j);
-          const float4 c0 = *(const float4*)(xbtab + (int64_t)key * 48 + old * 8);
-          const float2 c1 = *(const float2*)(xbtab + (int64_t)key * 48 + old * 8 + 4);
+          const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(seed, t, 1, j);
+          const float4 c0 = perm ? q_r1[q] : q_r0[q];
+          const float2 c1 = perm ? q_s1[q] : q_s0[q];
           nw = u < c0.x ? 0 : u < c0.y ? 1 : u < c0.z ? 2 : u < c0.w ? 3 : u < c1.x ? 4 : u < c1.y ? 5 : 0;
           const bool ob = old >= 1 && old <= 3, nbd = nw >= 1 && nw <= 3;
           const bool binding = !ob && nbd, unbinding = ob && !nbd;
@@ -813,16 +954,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           }
           if (can_bind) {
             site = dm.nk + thin * dm.ns + nsite;
-            atomicMin((unsigned int*)&sh.tbound[site - dm.nk], (unsigned int)j);
+            atomicMin((unsigned int*)(sh.tbound + (site - dm.nk)), (unsigned int)j);
           }
           if (binding && !can_bind) nw = 0;
           sh.xsite[j] = (short)site;
           sh.xst[j] = (unsigned char)nw;
           sh.xkey[j] = key;
-          if (site >= 0) sh.xnext[j] = (short)atomicExch(&sh.shead[site - dm.nk], j);
+          if (site >= 0) sh.xnext[j] = (short)atomicExch((int*)(sh.shead + (site - dm.nk)), j);
           // expected ATP / tear from the previous step's key (metrics_fn.py:297-335)
           if (old == 2 || old == 3) {
-            const float4 a = *(const float4*)(abstab + (int64_t)q_keyold[q] * 4);
+            const float4 a = q_ab[q];
             if (old == 3) m_atpe += a.x;
             m_tear += old == 3 ? a.z : a.y;
           }
@@ -850,7 +991,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
 
     // ---- metrics (metrics_fn.py:154-417) -----------------------------------
     if (L.metrics != nullptr || L.force != nullptr) {
-      const unsigned char* tmc_p = sh.tm + tmc * S;
+      const LDS unsigned char* tmc_p = sh.tm + tmc * S;
       const double kk = (double)sh.prm[P_THICK_K];
       const double ta = sh.prm[P_TITIN_A], tbp = sh.prm[P_TITIN_B], trest = sh.prm[P_TITIN_REST];
       // pass 1: node sums and extrema (wave partials), per-filament values
@@ -864,7 +1005,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         if (sl.thick[s]) {
           sk += dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
           wk += x - xpre[s];
-          double* fv = sh.fil + sl.fil[s] * NF;
+          LDS double* fv = sh.fil + sl.fil[s] * NF;
           if (sl.li[s] == 0) {
             const double e1 = 0.5 * kk * dd * dd;
             const double d0 = xpre[s] - (double)co[s];
@@ -888,7 +1029,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       mxk = wave_max(mxk, lane); nmk = wave_max(nmk, lane);
       mxn = wave_max(mxn, lane); nmn = wave_max(nmn, lane);
       if (lane == 0) {
-        double* wp = sh.wpart + wave * NWP;
+        LDS double* wp = sh.wpart + wave * NWP;
         wp[W_SUMK] = sk; wp[W_SUMN] = sn; wp[W_WORK] = wk;
         wp[W_MAXK] = mxk; wp[W_NMINK] = nmk; wp[W_MAXN] = mxn; wp[W_NMINN] = nmn;
       }
@@ -899,7 +1040,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         bool ov = false;
         if (on) {
           const float x = posf<REF64>(sh, dm.nk + s2);
-          ov = x >= G.overlap_lo && x <= G.overlap_hi && x > 0.0f;
+          ov = x >= ov_lo && x <= ov_hi && x > 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) wave_count(&sh.cnt[C_TM0 + k], st == k, lane);
@@ -911,7 +1052,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       double sumk = 0, sumn = 0;
 #pragma unroll
       for (int w = 0; w < NW; ++w) { sumk += sh.wpart[w * NWP + W_SUMK]; sumn += sh.wpart[w * NWP + W_SUMN]; }
-      const double mean_k = sumk / dm.nk, mean_n = sumn / S;
+      const double mean_k = sumk * inv_nk, mean_n = sumn * inv_S;
       // pass 2: population variances (jnp.std)
       double qk = 0, qn = 0;
 #pragma unroll
@@ -926,61 +1067,66 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       qn = wave_sum(qn, lane);
       if (lane == 0) { sh.wpart[wave * NWP + W_SQK] = qk; sh.wpart[wave * NWP + W_SQN] = qn; }
       __syncthreads();
-      if (tid == 0) {
-        double acc[NWP];
-#pragma unroll
-        for (int k = 0; k < NWP; ++k) acc[k] = 0.0;
-        acc[W_MAXK] = acc[W_NMINK] = acc[W_MAXN] = acc[W_NMINN] = -INFINITY;
-        // rolled loops: unrolled, every partial would be loaded at once
-#pragma unroll 1
-        for (int w = 0; w < NW; ++w) {
-          const double* wp = sh.wpart + w * NWP;
-#pragma unroll
-          for (int k = 0; k < NWP; ++k) {
-            const bool ismax = k == W_MAXK || k == W_NMINK || k == W_MAXN || k == W_NMINN;
-            acc[k] = ismax ? fmax(acc[k], wp[k]) : acc[k] + wp[k];
+      if (wave == 0) {
+        // lane k < NWP folds wave-partial column k, lanes 16.. fold the
+        // per-filament columns, lanes 32.. take (and reset) the counters;
+        // lane 0 then reads them across the wave and writes the row.
+        double v = 0.0;
+        if (lane < NWP) {
+          const bool ismax = lane == W_MAXK || lane == W_NMINK || lane == W_MAXN || lane == W_NMINN;
+          v = ismax ? -INFINITY : 0.0;
+          for (int w = 0; w < NW; ++w) {
+            const double x = sh.wpart[w * NWP + lane];
+            v = ismax ? fmax(v, x) : v + x;
           }
+        } else if (lane >= 16 && lane < 16 + NF) {
+          for (int f = 0; f < dm.Tk; ++f) v += sh.fil[f * NF + (lane - 16)];
+        } else if (lane >= 32 && lane < 32 + NC) {
+          v = (double)sh.cnt[lane - 32];
+          sh.cnt[lane - 32] = 0;
         }
-        double fv[NF] = {0, 0, 0, 0, 0};
-#pragma unroll 1
-        for (int f = 0; f < dm.Tk; ++f)
+        const double force = read_lane(v, 16 + F_FORCE);
+        const double work = force * (read_lane(v, W_WORK) * inv_nk);
+        const double atpe = read_lane(v, W_ATPE);
+        if (lane == 0) {
+          if (L.metrics) {
+            float c[NC];
 #pragma unroll
-          for (int k = 0; k < NF; ++k) fv[k] += sh.fil[f * NF + k];
-        int c[NC];
+            for (int k = 0; k < NC; ++k) c[k] = (float)read_lane(v, 32 + k);
+            const float nbound = c[C_XB0 + 1] + c[C_XB0 + 2] + c[C_XB0 + 3];
+            const float ov = c[C_OV];
+            const float invX = 1.0f / (float)X, invS = 1.0f / (float)S, invTk = 1.0f / (float)dm.Tk;
+            float* m = L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS;
+            m[0] = (float)force; m[1] = resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
+            m[5] = nbound;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) { c[k] = sh.cnt[k]; sh.cnt[k] = 0; }
-        const double Xd = (double)X, Sd = (double)S;
-        const double force = fv[F_FORCE];
-        const double nbound = (double)(c[C_XB0 + 1] + c[C_XB0 + 2] + c[C_XB0 + 3]);
-        const double work = force * (acc[W_WORK] / dm.nk);
-        const double ov = (double)c[C_OV];
-        if (L.metrics) {
-          float* m = L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS;
-          m[0] = (float)force; m[1] = resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
-          m[5] = (float)nbound;
-          for (int k = 0; k < 6; ++k) m[6 + k] = (float)c[C_XB0 + k];
-          m[12] = (float)(nbound / Xd);
-          for (int k = 0; k < 6; ++k) m[13 + k] = (float)(c[C_XB0 + k] / Xd);
-          for (int k = 0; k < 4; ++k) m[19 + k] = (float)c[C_TM0 + k];
-          for (int k = 0; k < 4; ++k) m[23 + k] = (float)(c[C_TM0 + k] / Sd);
-          m[27] = (float)(c[C_TM0 + 3] / Sd);
-          m[28] = (float)(c[C_OV2] / ov);
-          m[29] = (float)(c[C_OV3] / ov);
-          m[30] = (float)((c[C_OV2] + c[C_OV3]) / ov);
-          m[31] = (float)ov;
-          m[32] = (float)c[C_ATPC]; m[33] = (float)c[C_NEWLY];
-          m[34] = (float)mean_k; m[35] = (float)acc[W_MAXK]; m[36] = (float)(-acc[W_NMINK]);
-          m[37] = (float)sqrt(acc[W_SQK] / dm.nk);
-          m[38] = (float)mean_n; m[39] = (float)acc[W_MAXN]; m[40] = (float)(-acc[W_NMINN]);
-          m[41] = (float)sqrt(acc[W_SQN] / S);
-          m[42] = (float)(fv[F_E1] / dm.Tk); m[43] = (float)(fv[F_DE1] / dm.Tk);
-          m[44] = (float)(fv[F_TIT] / dm.Tk); m[45] = (float)(fv[F_DTIT] / dm.Tk);
-          m[46] = (float)work; m[47] = (float)(work / (double)dm.nk);
-          m[48] = (float)acc[W_ATPE]; m[49] = (float)acc[W_TEAR];
-          m[50] = (float)(acc[W_ATPE] > 0.01 ? work / acc[W_ATPE] : 0.0);
-          m[51] = (float)iters;
+            for (int k = 0; k < 6; ++k) m[6 + k] = c[C_XB0 + k];
+            m[12] = nbound * invX;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) m[13 + k] = c[C_XB0 + k] * invX;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) m[19 + k] = c[C_TM0 + k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) m[23 + k] = c[C_TM0 + k] * invS;
+            m[27] = c[C_TM0 + 3] * invS;
+            m[28] = c[C_OV2] / ov;
+            m[29] = c[C_OV3] / ov;
+            m[30] = (c[C_OV2] + c[C_OV3]) / ov;
+            m[31] = ov;
+            m[32] = c[C_ATPC]; m[33] = c[C_NEWLY];
+            m[34] = (float)mean_k; m[35] = (float)read_lane(v, W_MAXK); m[36] = (float)(-read_lane(v, W_NMINK));
+            m[37] = (float)sqrt(read_lane(v, W_SQK) * inv_nk);
+            m[38] = (float)mean_n; m[39] = (float)read_lane(v, W_MAXN); m[40] = (float)(-read_lane(v, W_NMINN));
+            m[41] = (float)sqrt(read_lane(v, W_SQN) * inv_S);
+            m[42] = (float)read_lane(v, 16 + F_E1) * invTk; m[43] = (float)read_lane(v, 16 + F_DE1) * invTk;
+            m[44] = (float)read_lane(v, 16 + F_TIT) * invTk; m[45] = (float)read_lane(v, 16 + F_DTIT) * invTk;
+            m[46] = (float)work; m[47] = (float)(work * inv_nk);
+            m[48] = (float)atpe; m[49] = (float)read_lane(v, W_TEAR);
+            m[50] = (float)(atpe > 0.01 ? work / atpe : 0.0);
+            m[51] = (float)iters;
+          }
+          if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
         }
-        if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
       }
     } else if (tid == 0) {
 #pragma unroll
@@ -1004,7 +1150,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     }
     int32_t* tbd = (int32_t*)(stb + SL.thin_bound);
     uint8_t* tms = stb + SL.tm;
-    const unsigned char* tcur = sh.tm + tmc * S;
+    const LDS unsigned char* tcur = sh.tm + tmc * S;
     for (int s = tid; s < S; s += BS) { tbd[s] = sh.tbound[s]; tms[s] = tcur[s]; }
     if (tid == 0) *(double*)(stb + SL.ls) = ls_carry;
   }
