@@ -31,6 +31,37 @@ HBM_PEAK_GBS = 8000.0
 # 2 * state (4N + X + 4X + S + 4S) + 12 (drivers) + 4*52 (metrics)
 #   = 2 * 10432 + 12 + 208 = 21084 B.
 BYTES_PER_SARC_STEP_2X2 = 21084
+# Measured HBM traffic of the step kernel on this workload (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, scaled by a plain copy kernel), bytes per
+# sarcomere-step.  The profile that holds it records its own command.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "run_kernel_2x2_sweep_traffic.md")
+COPY_PROBE_MIB = 256
+
+
+def measured_traffic_per_sarc_step():
+    """Read 'traffic_bytes_per_sarcomere_step: <float>' from the committed
+    traffic profile; None when it is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_sarcomere_step:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib=COPY_PROBE_MIB, reps=3):
+    """Plain device copy of a known size: the calibration dispatch for the
+    FETCH_SIZE / WRITE_SIZE counters (reads and writes mib MiB each)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_read": mib << 20, "bytes_written": mib << 20, "reps": reps}
 
 
 def conditions(B: int, T: int, seed: int = 20261203):
@@ -124,6 +155,8 @@ def main():
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--n-cg", type=int, default=6, help="CG iterations per Newton step")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/oracle/cpu legs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help=f"after timing, copy {COPY_PROBE_MIB} MiB on the device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
@@ -152,6 +185,8 @@ def main():
     value = B * K / (ms / 1e3)
     bytes_launch = BYTES_PER_SARC_STEP_2X2 * B * K
     achieved = bytes_launch / (ms / 1e3) / 1e9
+    tps = measured_traffic_per_sarc_step()
+    traffic = None if tps is None else tps * B * K
     out = {
         "metric": METRIC, "value": value, "unit": "sarcomere-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": ms / K, "higher_is_better": True,
@@ -162,9 +197,11 @@ def main():
                    "lattice": "2x2", "batch": B, "solver": args.mode, "block": bt.block,
                    "n_cg": args.n_cg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "host_s": t_host,
     }
+    if args.copy_probe:
+        out["copy_probe"] = copy_probe()
     if not args.no_extras:
         pv, psample = plain_gpu(cfg)
         out["plain_gpu"] = {"value": pv, "unit": "sarcomere-steps/s", "sample": psample}

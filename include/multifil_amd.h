@@ -113,7 +113,7 @@ typedef struct mf_launch {
   double* force;                /* [B][t_stride] FP64 M-line force or NULL */
   int32_t n_lattices, B, t_stride, u_stride;
   mf_solver_cfg cfg;
-  int32_t block;                /* threads per workgroup: 256, 512 or 1024 */
+  int32_t block;                /* threads per workgroup: 256, 512, 768 or 1024 */
   int32_t t_base;               /* added to every instance's t0            */
   const mf_lattice* lattices_host; /* optional HOST copy of `lattices`; when
                                    NULL the library reads the device copy  */

@@ -164,10 +164,14 @@ __device__ __forceinline__ int p_xb(int pk) { return pk & 0xffff; }
 
 // Integer metric counters (LDS, reset by thread 0 after each step's output).
 enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, NC = 16 };
-// Per-thick-filament metric values written by the owning lane.
-enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF };
-// Per-wave partial sums: pass 1 (node sums / maxima, XB expectations), pass 2.
-enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_ATPE, W_TEAR, W_SQK, W_SQN, NWP };
+// Per-thick-filament metric values written by the owning lane; the first
+// NF_SUM are summed over filaments, F_X0 / F_XT hold the pre-solve positions
+// of the M-line and tip crowns.
+enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF_SUM, F_X0 = NF_SUM, F_XT, NF };
+// Per-wave partial sums: pass 1 (node sums / maxima, XB expectations), pass 2,
+// and the pre-solve sum of thick positions (work = sum x_new - sum x_pre).
+enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_ATPE, W_TEAR, W_SQK, W_SQN,
+                W_XPRE, NWP };
 
 // All LDS arrays of one workgroup.
 template <bool REF64>
@@ -638,9 +642,9 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
 // Nearest reachable site (geometry.py:60-175): search from crown+13, mask
 // padding slots and sites at x <= 0, first minimum wins; returns the site
 // index along the thin filament and the float32 axial offset from the crown.
-// Widths up to 32 run fully unrolled so every candidate load is in flight at
-// once.
-constexpr int MAXW_UNROLLED = 32;
+// Candidates are gathered in chunks of NEAR_CHUNK so a chunk's LDS loads are
+// in flight together without holding all W positions in registers.
+constexpr int NEAR_CHUNK = 8;
 template <bool REF64>
 __device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, int slot, int thin,
                                              int crown, int& nearest, float& xdist) {
@@ -651,22 +655,15 @@ __device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims
   const int tb = dm.nk + thin * dm.ns;
   float best = INFINITY;
   int bj = 0;
-  if (dm.W <= MAXW_UNROLLED) {
-    float sp[MAXW_UNROLLED];
+  for (int j0 = 0; j0 < dm.W; j0 += NEAR_CHUNK) {
+    float sp[NEAR_CHUNK];
 #pragma unroll
-    for (int j = 0; j < MAXW_UNROLLED; ++j) sp[j] = j < dm.W ? posf<REF64>(sh, tb + cs[j]) : 0.0f;
+    for (int c = 0; c < NEAR_CHUNK; ++c) sp[c] = j0 + c < dm.W ? posf<REF64>(sh, tb + cs[j0 + c]) : 0.0f;
 #pragma unroll
-    for (int j = 0; j < MAXW_UNROLLED; ++j) {
-      const float dd = fabsf(sp[j] - head);
-      const bool ok = (j < cnt) && (sp[j] > 0.0f);
-      if (ok && dd < best) { best = dd; bj = j; }
-    }
-  } else {
-    for (int j = 0; j < dm.W; ++j) {
-      const float spj = posf<REF64>(sh, tb + cs[j]);
-      const float dd = fabsf(spj - head);
-      const bool ok = (j < cnt) && (spj > 0.0f);
-      if (ok && dd < best) { best = dd; bj = j; }
+    for (int c = 0; c < NEAR_CHUNK; ++c) {
+      const float dd = fabsf(sp[c] - head);
+      const bool ok = (j0 + c < cnt) && (sp[c] > 0.0f);
+      if (ok && dd < best) { best = dd; bj = j0 + c; }
     }
   }
   nearest = cs[bj];
@@ -697,7 +694,10 @@ template <int BS, int MS, int MX, bool REF64>
 __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   extern __shared__ __align__(16) char smem[];
   constexpr int NW = BS / WAVE;
-  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+  // the wave index is uniform: readfirstlane keeps it, and everything derived
+  // from it (chain ownership), in scalar registers
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
   const int b = blockIdx.x;
   // descriptors are read in place (scalar loads); a by-value copy of these
   // structs would live in scratch
@@ -842,7 +842,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     const bool kin = step >= 0;
     const int t = t_first + (kin ? step : 0);
     const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
-    double xpre[MS];
     float z_solve, d_solve;
     if (!kin) {
       // Nearest update, then a solve against the parameter row's z_line and
@@ -855,8 +854,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
       d_solve = sh.prm[P_LS];
-#pragma unroll
-      for (int s = 0; s < MS; ++s) xpre[s] = 0.0;
     } else {
       const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
       z_solve = zl;
@@ -868,8 +865,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         __syncthreads();
       }
-#pragma unroll
-      for (int s = 0; s < MS; ++s) xpre[s] = sl.node[s] >= 0 ? posd<REF64>(sh, sl.node[s]) : 0.0;
 
       // ---- K1: TM transitions (transitions.py:516-625) + nearest sites -------
       const LDS unsigned char* tmo = sh.tm + tmc * S;
@@ -926,6 +921,23 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       __syncthreads();
       tmc ^= 1;
       const LDS unsigned char* tmc_p = sh.tm + tmc * S;
+      // pre-solve thick positions the metrics need (work, energy changes).
+      // Written after K1's barrier: wave 0 may still be reading the previous
+      // step's partials until then.
+      {
+        double wxp = 0.0;
+#pragma unroll
+        for (int s = 0; s < MS; ++s) {
+          const int n = sl.node[s];
+          if (n < 0 || !sl.thick[s]) continue;
+          const double xp = posd<REF64>(sh, n);
+          wxp += xp;
+          if (sl.li[s] == 0) sh.fil[sl.fil[s] * NF + F_X0] = xp;
+          if (sl.li[s] == dm.nc - 1) sh.fil[sl.fil[s] * NF + F_XT] = xp;
+        }
+        wxp = wave_sum(wxp, lane);
+        if (lane == 0) sh.wpart[wave * NWP + W_XPRE] = wxp;
+      }
 
       // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
       float m_atpe = 0.0f, m_tear = 0.0f;
@@ -1004,17 +1016,17 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const double dd = x - (double)co[s];
         if (sl.thick[s]) {
           sk += dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
-          wk += x - xpre[s];
+          wk += x;
           LDS double* fv = sh.fil + sl.fil[s] * NF;
           if (sl.li[s] == 0) {
             const double e1 = 0.5 * kk * dd * dd;
-            const double d0 = xpre[s] - (double)co[s];
+            const double d0 = fv[F_X0] - (double)co[s];
             fv[F_FORCE] = dd * kk;
             fv[F_E1] = e1;
             fv[F_DE1] = e1 - 0.5 * kk * d0 * d0;
           }
           if (sl.li[s] == dm.nc - 1) {
-            const double an = (double)zl - x, ao = (double)zl - xpre[s];
+            const double an = (double)zl - x, ao = (double)zl - fv[F_XT];
             const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
             const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
             const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
@@ -1079,14 +1091,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             const double x = sh.wpart[w * NWP + lane];
             v = ismax ? fmax(v, x) : v + x;
           }
-        } else if (lane >= 16 && lane < 16 + NF) {
+        } else if (lane >= 16 && lane < 16 + NF_SUM) {
           for (int f = 0; f < dm.Tk; ++f) v += sh.fil[f * NF + (lane - 16)];
         } else if (lane >= 32 && lane < 32 + NC) {
           v = (double)sh.cnt[lane - 32];
           sh.cnt[lane - 32] = 0;
         }
         const double force = read_lane(v, 16 + F_FORCE);
-        const double work = force * (read_lane(v, W_WORK) * inv_nk);
+        const double work = force * ((read_lane(v, W_WORK) - read_lane(v, W_XPRE)) * inv_nk);
         const double atpe = read_lane(v, W_ATPE);
         if (lane == 0) {
           if (L.metrics) {
@@ -1174,6 +1186,7 @@ Variant make_variant() {
 const Variant* variants(int& n) {
   static const Variant v[] = {
       make_variant<1024, 2, 1, false>(), make_variant<1024, 2, 1, true>(),
+      make_variant<768, 2, 1, false>(),  make_variant<768, 2, 1, true>(),
       make_variant<1024, 8, 4, false>(), make_variant<1024, 8, 4, true>(),
       make_variant<512, 4, 2, false>(),  make_variant<512, 4, 2, true>(),
       make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
@@ -1278,7 +1291,8 @@ int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes) {
 
 int mf_run(const mf_launch* L, void* stream) {
   if (!L) return fail("null launch");
-  if (L->block != 256 && L->block != 512 && L->block != 1024) return fail("block must be 256, 512 or 1024");
+  if (L->block != 256 && L->block != 512 && L->block != 768 && L->block != 1024)
+    return fail("block must be 256, 512, 768 or 1024");
   if (L->cfg.n_cg < 1 || L->cfg.n_cg > 64) return fail("n_cg out of range");
   std::vector<mf_lattice> lats;
   if (int e = fetch_lattices(L, lats)) return e;
