@@ -184,15 +184,17 @@ struct Shared {
   LDS float* kx;         // [X]   XB axial stiffness df/dx
   LDS short* xsite;      // [X]   DOF index of the bound site, -1 unbound
   LDS short* xnext;      // [X]   next XB in the site's bound list
-  LDS int* xkey;         // [X]   table row of the last kinetics step
+  LDS short* xkey;       // [X]   table row of the last kinetics step (< 2 * MF_MAX_BINS)
+  LDS short* xslot;      // [X]   (thin, face) slot | target-valid << 15
+  LDS float* org;        // [N]   displacement origin: crown offset / cumulative thin rest
   LDS unsigned char* xst;  // [X] XB state
   LDS int* shead;        // [S]   first XB bound to the site, -1 none
   LDS int* tbound;       // [S]   thin.bound_to
   LDS unsigned char* tm; // [2][S] TM state (double buffered)
   LDS short* fsites;     // [Tn*nf*W] candidate sites per (thin, face)
   LDS short* fcount;     // [Tn*nf]
-  LDS short* tprev;      // [S] same-chain neighbours
-  LDS short* tnext;      // [S]
+  LDS signed char* tprev;  // [S] same-chain neighbours as offsets along the filament
+  LDS signed char* tnext;  // [S] (0: none, the site references itself)
   LDS float* tmcum;      // [27*16]
   LDS float* edges;      // [nb+1]
   LDS float* prm;        // [MF_N_PARAMS]
@@ -205,7 +207,7 @@ struct Shared {
 // LDS carve-up.  One routine computes the byte offsets (host: the dynamic
 // LDS size; device: the typed pointers), so the two cannot disagree.
 struct LdsOffsets {
-  int64_t x, pv, fx, kx, xsite, xnext, xkey, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
+  int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
       edges, prm, red, wpart, fil, cnt, total;
 };
 template <int BS, bool REF64>
@@ -220,15 +222,17 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.kx = take(4LL * d.X);
   L.xsite = take(2LL * d.X);
   L.xnext = take(2LL * d.X);
-  L.xkey = take(4LL * d.X);
+  L.xkey = take(2LL * d.X);
+  L.xslot = take(2LL * d.X);
+  L.org = take(4LL * d.N);
   L.xst = take(d.X);
   L.shead = take(4LL * d.S);
   L.tbound = take(4LL * d.S);
   L.tm = take(2LL * d.S);
   L.fsites = take(2LL * d.Tn * d.nf * d.W);
   L.fcount = take(2LL * d.Tn * d.nf);
-  L.tprev = take(2LL * d.S);
-  L.tnext = take(2LL * d.S);
+  L.tprev = take(d.S);
+  L.tnext = take(d.S);
   L.tmcum = take(4 * 27 * 16);
   L.edges = take(4LL * (d.nb + 1));
   L.prm = take(4 * MF_N_PARAMS);
@@ -254,15 +258,17 @@ __device__ __forceinline__ Shared<REF64> carve(LDS char* base, const Dims& d) {
   t.kx = (LDS float*)(base + o.kx);
   t.xsite = (LDS short*)(base + o.xsite);
   t.xnext = (LDS short*)(base + o.xnext);
-  t.xkey = (LDS int*)(base + o.xkey);
+  t.xkey = (LDS short*)(base + o.xkey);
+  t.xslot = (LDS short*)(base + o.xslot);
+  t.org = (LDS float*)(base + o.org);
   t.xst = (LDS unsigned char*)(base + o.xst);
   t.shead = (LDS int*)(base + o.shead);
   t.tbound = (LDS int*)(base + o.tbound);
   t.tm = (LDS unsigned char*)(base + o.tm);
   t.fsites = (LDS short*)(base + o.fsites);
   t.fcount = (LDS short*)(base + o.fcount);
-  t.tprev = (LDS short*)(base + o.tprev);
-  t.tnext = (LDS short*)(base + o.tnext);
+  t.tprev = (LDS signed char*)(base + o.tprev);
+  t.tnext = (LDS signed char*)(base + o.tnext);
   t.tmcum = (LDS float*)(base + o.tmcum);
   t.edges = (LDS float*)(base + o.edges);
   t.prm = (LDS float*)(base + o.prm);
@@ -291,6 +297,22 @@ __device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave
 #pragma unroll
   for (int w = 0; w < Red<BS>::NW; ++w) t += r[w];
   return t;
+}
+// Two sums with one barrier (consumes two ring rows; a row is rewritten only
+// after another reduction's barrier, by which time every wave has read it).
+template <int BS>
+__device__ __forceinline__ void wg_sum2(Red<BS>& rd, float& a, float& b, int lane, int wave) {
+  LDS float* ra = rd.row();
+  LDS float* rb = rd.row();
+  const float sa = wave_sum(a, lane);
+  const float sb = wave_sum(b, lane);
+  if (lane == 0) { ra[wave] = sa; rb[wave] = sb; }
+  __syncthreads();
+  float ta = 0.0f, tb = 0.0f;
+#pragma unroll
+  for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; }
+  a = ta;
+  b = tb;
 }
 template <int BS>
 __device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave) {
@@ -555,15 +577,52 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
   }
 }
 
-// Unrolled preconditioned CG on A = -J from x0 = 0 (solver.py:329-366).  The
-// first pass's beta is 0; the last pass skips the unused preconditioner
-// apply and rz reduction.
+// Preconditioned CG on A = -J from x0 = 0, n_cg iterations (solver.py:329-366).
+//
+// Reference mode runs the textbook recurrence the reference unrolls: two
+// reductions (rz, pAp) and three barriers per iteration.  Refined mode runs
+// the Chronopoulos-Gear form of the same method (identical iterates in exact
+// arithmetic): u = M r and w = A u are formed first and (r.u, w.u) reduced
+// together, so each iteration costs one fused reduction and two barriers;
+// p and s = A p follow from u, w by the beta recurrence.  Refined mode only
+// needs the correction to converge: the FP64 residual decides when to stop.
 template <int BS, int MS, bool REF64>
 __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
                                     const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
                                     int n_cg, float (&x)[MS], Red<BS>& rd, int lane, int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
   const float ikk = 1.0f / kk, ikn = 1.0f / kn;
+  if constexpr (REF64) {
+    float r[MS], u[MS], w[MS], p[MS], q[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; q[s] = 0.0f; }
+    float gamma = 0.0f, alpha = 1.0f;
+    for (int it = 0;; ++it) {
+      precond<MS>(r, u, sl, lane, ikk, ikn);
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (sl.node[s] >= 0) sh.pv[sl.node[s]] = u[s];
+      __syncthreads();
+      neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, w);
+      float g = 0.0f, dl = 0.0f;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) { g += r[s] * u[s]; dl += w[s] * u[s]; }
+      wg_sum2<BS>(rd, g, dl, lane, wave);
+      const float beta = it == 0 ? 0.0f : g / (gamma + 1e-30f);
+      const float den = it == 0 ? dl : dl - beta * g / alpha;
+      alpha = g / (den + 1e-30f);
+      gamma = g;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        p[s] = u[s] + beta * p[s];
+        q[s] = w[s] + beta * q[s];
+        x[s] += alpha * p[s];
+        r[s] -= alpha * q[s];
+      }
+      if (it >= n_cg - 1) break;
+    }
+    return;
+  }
   float r[MS], z[MS], p[MS], Ap[MS];
 #pragma unroll
   for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; }
@@ -727,22 +786,18 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   for (int i = tid; i < 27 * 16; i += BS) sh.tmcum[i] = tab[TAB_TM + i];
   for (int i = tid; i < dm.Tn * dm.nf * dm.W; i += BS) sh.fsites[i] = (short)li[G.face_sites + i];
   for (int i = tid; i < dm.Tn * dm.nf; i += BS) sh.fcount[i] = (short)li[G.face_count + i];
-  for (int i = tid; i < S; i += BS) { sh.tprev[i] = (short)li[G.tm_prev + i]; sh.tnext[i] = (short)li[G.tm_next + i]; }
+  for (int i = tid; i < S; i += BS) {
+    const int k = i % dm.ns;  // host checks |neighbour - k| <= 127
+    sh.tprev[i] = (signed char)(li[G.tm_prev + i] - k);
+    sh.tnext[i] = (signed char)(li[G.tm_next + i] - k);
+  }
   for (int i = tid; i < NC; i += BS) sh.cnt[i] = 0;
 
-  // this thread's XBs: target thin, (thin, face) slot, validity
-  int q_thin[MX], q_slot[MX];
-  bool q_valid[MX];
-#pragma unroll
-  for (int q = 0; q < MX; ++q) {
-    const int j = tid + q * BS;
-    q_thin[q] = 0; q_slot[q] = 0; q_valid[q] = false;
-    if (j < X) {
-      q_thin[q] = li[G.xb_thin + j];
-      q_slot[q] = li[G.xb_slot + j];
-      q_valid[q] = li[G.xb_valid + j] != 0;
-    }
-  }
+  // per-XB (thin, face) slot with the target-valid flag; displacement origins
+  for (int j = tid; j < X; j += BS)
+    sh.xslot[j] = (short)(li[G.xb_slot + j] | (li[G.xb_valid + j] != 0 ? 0x8000 : 0));
+  for (int i = tid; i < dm.N; i += BS)
+    sh.org[i] = i < dm.nk ? lf[G.crown_offsets + i] : lf[G.thin_rest_cum + (i - dm.nk)];
 
   // chain-major slot map
   Slots<MS> sl;
@@ -770,12 +825,11 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     }
     sl.nch = k;
   }
-  // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1};
-  // co: crown offset (thick) or cumulative thin rest (thin), the displacement origins
-  float rA[MS], rB[MS], co[MS];
+  // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1}
+  float rA[MS], rB[MS];
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
-    rA[s] = rB[s] = co[s] = 0.0f;
+    rA[s] = rB[s] = 0.0f;
     const int n = sl.node[s];
     if (n < 0) continue;
     const int i = sl.li[s];
@@ -783,12 +837,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       const float* cr = lf + G.crown_rests + (int64_t)sl.fil[s] * dm.nc;
       rA[s] = cr[i];
       rB[s] = i < dm.nc - 1 ? cr[i + 1] : 0.0f;
-      co[s] = lf[G.crown_offsets + n];
     } else {
       const float* br = lf + G.binding_rests + (int64_t)sl.fil[s] * dm.ns;
       rA[s] = br[i];
       rB[s] = i > 0 ? br[i - 1] : 0.0f;
-      co[s] = lf[G.thin_rest_cum + (n - dm.nk)];
     }
   }
 
@@ -806,7 +858,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     for (int j = tid; j < X; j += BS) {
       const int bt = xbb[j];
       sh.xsite[j] = (short)(bt >= 0 ? dm.nk + li[G.xb_thin + j] * dm.ns + bt : -1);
-      sh.xkey[j] = xbk[j];
+      sh.xkey[j] = (short)xbk[j];
       sh.xst[j] = xbs[j];
     }
     const int32_t* tbd = (const int32_t*)(stb + SL.thin_bound);
@@ -848,8 +900,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       // lattice_spacing (the reference's run() leaves those at their defaults).
       for (int j = tid; j < X; j += BS) {
         int nr; float xd;
-        nearest_site<REF64>(sh, dm, li[G.xb_slot + j], li[G.xb_thin + j], j / dm.nx, nr, xd);
-        sh.xkey[j] = bin_of(sh.edges, nb, inv_w, xd);  // tm == 0 everywhere: permissiveness 0
+        const int slot = sh.xslot[j] & 0x7fff;
+        nearest_site<REF64>(sh, dm, slot, slot / dm.nf, j / dm.nx, nr, xd);
+        sh.xkey[j] = (short)bin_of(sh.edges, nb, inv_w, xd);  // tm == 0 everywhere: permissiveness 0
       }
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
@@ -870,10 +923,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       const LDS unsigned char* tmo = sh.tm + tmc * S;
       LDS unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
       for (int s = tid; s < S; s += BS) {
-        const int row = (s / dm.ns) * dm.ns, i = s - row;
         const int pv = sh.tprev[s], nv = sh.tnext[s];
-        const int sp_ = tmo[row + pv], sn_ = tmo[row + nv];
-        const bool pr = pv != i, nr = nv != i;
+        const int sp_ = tmo[s + pv], sn_ = tmo[s + nv];
+        const bool pr = pv != 0, nr = nv != 0;
         const int n2 = (pr && sp_ == 2) + (nr && sn_ == 2);
         const int n3 = (pr && sp_ == 3) + (nr && sn_ == 3);
         const int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
@@ -906,7 +958,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const int old = sh.xst[j];
           const int keyold = sh.xkey[j];
           float xd;
-          nearest_site<REF64>(sh, dm, q_slot[q], q_thin[q], j / dm.nx, q_near[q], xd);
+          const int slot = sh.xslot[j] & 0x7fff;
+          const int thin = slot / dm.nf;
+          nearest_site<REF64>(sh, dm, slot, thin, j / dm.nx, q_near[q], xd);
+          q_near[q] += thin * dm.ns;  // thin-global site index from here on
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
           const float* r0 = xbtab + (int64_t)bn * 48 + old * 8;
           const float* r1 = xbtab + (int64_t)(nb + bn) * 48 + old * 8;
@@ -914,7 +969,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
           if (old == 2 || old == 3) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
           q_bin[q] = bn;
-          q_occ[q] = sh.tbound[q_thin[q] * dm.ns + q_near[q]] >= 0;
+          q_occ[q] = sh.tbound[q_near[q]] >= 0;
           q_old[q] = old;
         }
       }
@@ -947,9 +1002,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const bool on = j < X;
         int nw = 0, old = 0;
         if (on) {
-          const int thin = q_thin[q];
-          const int nsite = q_near[q];
-          const bool perm = (tmc_p[thin * dm.ns + nsite] == 3) && q_valid[q];
+          const int gsite = q_near[q];
+          const bool perm = (tmc_p[gsite] == 3) && (sh.xslot[j] < 0);
           const int key = (perm ? nb : 0) + q_bin[q];
           old = q_old[q];
           const float u = U ? U[(int64_t)t * L.u_stride + S + j] : philox_uniform(seed, t, 1, j);
@@ -965,13 +1019,13 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             site = -1;
           }
           if (can_bind) {
-            site = dm.nk + thin * dm.ns + nsite;
+            site = dm.nk + gsite;
             atomicMin((unsigned int*)(sh.tbound + (site - dm.nk)), (unsigned int)j);
           }
           if (binding && !can_bind) nw = 0;
           sh.xsite[j] = (short)site;
           sh.xst[j] = (unsigned char)nw;
-          sh.xkey[j] = key;
+          sh.xkey[j] = (short)key;
           if (site >= 0) sh.xnext[j] = (short)atomicExch((int*)(sh.shead + (site - dm.nk)), j);
           // expected ATP / tear from the previous step's key (metrics_fn.py:297-335)
           if (old == 2 || old == 3) {
@@ -1013,14 +1067,15 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const int n = sl.node[s];
         if (n < 0) continue;
         const double x = posd<REF64>(sh, n);
-        const double dd = x - (double)co[s];
+        const double o = (double)sh.org[n];
+        const double dd = x - o;
         if (sl.thick[s]) {
           sk += dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
           wk += x;
           LDS double* fv = sh.fil + sl.fil[s] * NF;
           if (sl.li[s] == 0) {
             const double e1 = 0.5 * kk * dd * dd;
-            const double d0 = fv[F_X0] - (double)co[s];
+            const double d0 = fv[F_X0] - o;
             fv[F_FORCE] = dd * kk;
             fv[F_E1] = e1;
             fv[F_DE1] = e1 - 0.5 * kk * d0 * d0;
@@ -1071,7 +1126,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       for (int s = 0; s < MS; ++s) {
         const int n = sl.node[s];
         if (n < 0) continue;
-        const double dd = posd<REF64>(sh, n) - (double)co[s];
+        const double dd = posd<REF64>(sh, n) - (double)sh.org[n];
         if (sl.thick[s]) qk += (dd - mean_k) * (dd - mean_k);
         else qn += (dd - mean_n) * (dd - mean_n);
       }

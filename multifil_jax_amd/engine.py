@@ -115,6 +115,11 @@ class LatticePack:
             # The face-table row of an XB must reproduce its own candidate row.
             if not np.array_equal(fs.reshape(-1, width)[slot], t.xb_to_site_indices):
                 raise ValueError("xb_to_site_indices is not face_to_sites[thin, face]")
+            # TM neighbours travel to the device as offsets along the filament
+            k = np.arange(t.n_sites)[None, :]
+            for nb_ in (t.tm_prev_neighbor, t.tm_next_neighbor):
+                if np.any(np.abs(nb_ - k) > 127) or np.any(nb_ < 0) or np.any(nb_ >= t.n_sites):
+                    raise ValueError("TM neighbours must lie within 127 sites on the same filament")
             co = t.crown_offsets.astype(np.float32)
             d = _lib.MfLattice(
                 n_thick=t.n_thick, n_crowns=t.n_crowns, n_thin=t.n_thin, n_sites=t.n_sites,
@@ -138,6 +143,13 @@ class LatticePack:
 def _torch():
     import torch
     return torch
+
+
+def default_block(topos) -> int:
+    """Workgroup size for a set of lattices: one wave per filament chain when
+    the chains fit in 12 waves (768 threads, 168 VGPRs per lane), else 1024."""
+    nch = max(t.n_thick + t.n_thin for t in topos)
+    return 768 if nch <= 12 else 1024
 
 
 class Batch:
@@ -198,7 +210,7 @@ class Batch:
         self.d_force = torch.zeros((self.B, self.T), dtype=torch.float64, device=dev)
         self._inst_cache = {}
         self.d_inst = None
-        self.block = 1024
+        self.block = default_block(pack.topos)
         self.initialized = np.zeros(self.B, dtype=bool)
         self._table_rows = None      # (B, 2) pCa / spacing the tables were built for
         self._table_dt = None
@@ -259,13 +271,15 @@ class Batch:
         return [(t0 + a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
     def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
-            block: Optional[int] = None, sync: bool = True):
+            block: Optional[int] = None, sync: bool = True, kernel_events=None):
         """Advance every instance from step t0 by n_steps (the first call
         equilibrates each instance before its first step, as the reference's
         create_and_equilibrate does).
 
         uniforms: optional device float32 tensor (B, T, S+X) of injected draws
         (TM sites first, then XBs) instead of the device Philox stream.
+        kernel_events: optional (start, end) torch.cuda.Event pair recorded on
+        the launch stream immediately around the step-kernel launches.
         """
         cfg = cfg or SolverConfig()
         if block is not None:
@@ -294,7 +308,11 @@ class Batch:
                 _lib.check(self.lib.mf_tables(C.byref(L), stream), "mf_tables")
                 self._table_rows = rows.copy()
                 self._table_dt = cfg.dt
+            if kernel_events is not None and s0 == segs[0][0]:
+                kernel_events[0].record(_torch().cuda.current_stream(self.device))
             _lib.check(self.lib.mf_run(C.byref(L), stream), "mf_run")
+            if kernel_events is not None and s0 == segs[-1][0]:
+                kernel_events[1].record(_torch().cuda.current_stream(self.device))
             self.initialized[:] = True
         if sync:
             _torch().cuda.synchronize(self.device)

@@ -1,0 +1,59 @@
+"""Host-side variant selection: every lattice the workloads use has a kernel
+variant that fits LDS and the register slots at the engine's default block
+(no GPU needed: mf_run_lds_bytes is host logic)."""
+import ctypes as C
+
+import pytest
+
+from multifil_jax_amd import _lib
+from multifil_jax_amd.engine import LatticePack, default_block
+from multifil_jax_amd.topology import SarcTopology
+
+LDS_PER_CU = 160 * 1024
+
+
+@pytest.fixture(scope="module")
+def lib():
+    try:
+        return _lib.load()
+    except (RuntimeError, OSError) as e:
+        pytest.skip(f"engine library not built: {e}")
+
+
+def _lds(lib, topos, mode, block):
+    pack = LatticePack(topos)
+    L = _lib.MfLaunch()
+    L.n_lattices = len(topos)
+    L.B = 1
+    L.block = block
+    L.cfg.mode = mode
+    L.cfg.n_cg = 6
+    L.lattices_host = C.addressof(pack.desc_host)
+    out = C.c_int64(0)
+    rc = lib.mf_run_lds_bytes(C.byref(L), C.byref(out))
+    return rc, out.value
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+@pytest.mark.parametrize("mode", [_lib.MF_SOLVER_REFERENCE, _lib.MF_SOLVER_REFINED])
+def test_variant_fits(lib, skeletal, n, mode):
+    s, d = skeletal
+    topo = SarcTopology.create(n, n, s, d)
+    rc, lds = _lds(lib, [topo], mode, default_block([topo]))
+    assert rc == 0, lib.mf_last_error().decode()
+    assert 0 < lds <= LDS_PER_CU
+
+
+def test_mixed_lattices_share_a_variant(lib, skeletal):
+    s, d = skeletal
+    topos = [SarcTopology.create(n, n, s, d) for n in (2, 3, 4)]
+    rc, lds = _lds(lib, topos, _lib.MF_SOLVER_REFINED, default_block(topos))
+    assert rc == 0, lib.mf_last_error().decode()
+    assert lds <= LDS_PER_CU
+
+
+def test_bad_block_is_rejected(lib, skeletal):
+    s, d = skeletal
+    topo = SarcTopology.create(2, 2, s, d)
+    rc, _ = _lds(lib, [topo], _lib.MF_SOLVER_REFINED, 640)
+    assert rc != 0
