@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 1
+#define MF_ABI_VERSION 2
 #define MF_N_PARAMS 44      /* DynamicParams fields, params.py:73-553 order */
 #define MF_N_METRICS 52     /* metrics_fn.py:343-417 order                */
 #define MF_MAX_BINS 256     /* n_xb_bins upper bound                       */
@@ -48,6 +48,11 @@ extern "C" {
 /* Instance flags. */
 #define MF_INST_INIT    1   /* run create_and_equilibrate before step 0     */
 #define MF_INST_DYNAMIC 2   /* dynamic lattice spacing (K_lat, nu)           */
+
+/* Launch flags. */
+#define MF_LAUNCH_DYNAMIC 1 /* some instance has MF_INST_DYNAMIC: run the
+                               kernels that carry the bordered (N+1) solve
+                               of solver.py:599-656                        */
 
 /* One lattice (SarcTopology.create output, packed).  All offsets are in
  * elements into the float / int32 lattice blobs. */
@@ -115,6 +120,8 @@ typedef struct mf_launch {
   mf_solver_cfg cfg;
   int32_t block;                /* threads per workgroup: 256, 512, 768 or 1024 */
   int32_t t_base;               /* added to every instance's t0            */
+  int32_t launch_flags;         /* MF_LAUNCH_*                             */
+  int32_t pad;
   const mf_lattice* lattices_host; /* optional HOST copy of `lattices`; when
                                    NULL the library reads the device copy  */
 } mf_launch;

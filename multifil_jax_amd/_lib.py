@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmultifil_amd.so")
 
-MF_ABI_VERSION = 1
+MF_ABI_VERSION = 2
 MF_N_PARAMS = 44
 MF_N_METRICS = 52
 MF_MAX_BINS = 256
@@ -20,6 +20,7 @@ MF_SOLVER_REFERENCE = 0
 MF_SOLVER_REFINED = 1
 MF_INST_INIT = 1
 MF_INST_DYNAMIC = 2
+MF_LAUNCH_DYNAMIC = 1
 
 EXPORTED = ("mf_abi_version", "mf_last_error", "mf_state_bytes", "mf_table_floats",
             "mf_tables", "mf_run", "mf_run_lds_bytes")
@@ -55,7 +56,8 @@ class MfLaunch(C.Structure):
                 ("metrics", C.c_void_p), ("force", C.c_void_p),
                 ("n_lattices", C.c_int32), ("B", C.c_int32), ("t_stride", C.c_int32),
                 ("u_stride", C.c_int32), ("cfg", MfSolverCfg), ("block", C.c_int32),
-                ("t_base", C.c_int32), ("lattices_host", C.c_void_p)]
+                ("t_base", C.c_int32), ("launch_flags", C.c_int32), ("pad", C.c_int32),
+                ("lattices_host", C.c_void_p)]
 
 
 _lib = None

@@ -60,51 +60,32 @@ __host__ __device__ inline StateLayout state_layout(int Tk, int nc, int Tn, int 
 // ---------------------------------------------------------------------------
 // transition tables: one matrix per thread
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_max) {
-  const int per = 27 + 4 * nb_max;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = (int)(gid / per);
-  const int m = (int)(gid % per);
-  if (b >= L.B) return;
-  const mf_instance I = L.instances[b];
-  const mf_lattice G = L.lattices[I.lattice];
-  const int nb = G.n_bins;
-  const float* prm = L.params + (int64_t)b * MF_N_PARAMS;
-  const float* drv = L.drivers + ((int64_t)b * L.t_stride + I.t0 + L.t_base) * 3;
-  const float pCa = drv[1];
-  float d = drv[2];
-  if (I.flags & MF_INST_DYNAMIC) {
-    const StateLayout SL = state_layout(G.n_thick, G.n_crowns, G.n_thin, G.n_sites, G.n_xb_per_crown);
-    if (!(I.flags & MF_INST_INIT)) d = (float)*(const double*)(L.state + I.state_off + SL.ls);
+// TM row block `cfg` (27 neighbour configurations): FP64 expm of Q dt, stored
+// as FP32 cumulative rows (transitions.py:434-513, 99-190).
+__device__ __forceinline__ void tm_table_entry(const float* prm, double pCa, double dt, int cfg, float* tab) {
+  double Q[4][4];
+  tm_generator<double>(prm, pCa, cfg, Q);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
+  expm_ref<4, double>(Q);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double c = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + cfg * 16 + i * 4 + j] = (float)c; }
   }
-  float* tab = L.tables + I.table_off;
-  if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
-  if (m < 27) {
-    double Q[4][4];
-    tm_generator<double>(prm, (double)pCa, m, Q);
-    const double dt = L.cfg.dt;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
-    expm_ref<4, double>(Q);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      double c = 0.0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + m * 16 + i * 4 + j] = (float)c; }
-    }
-    return;
-  }
-  int key = m - 27;
-  const bool absorbing = key >= 2 * nb_max;
-  if (absorbing) key -= 2 * nb_max;
-  if (key >= 2 * nb) return;
-  const float* centers = L.lat_f32 + G.bin_centers;
+}
+
+// XB key (perm * nb + bin) at spacing d: the 6 cumulative rows of the
+// sampling table (transitions.py:632-922), or, for the absorbing chain
+// (rows 0 and 4 zeroed, transitions.py:1029-1078), the three entries the
+// metrics use.
+__device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, double d, double dt, double center,
+                                      int key, int nb, bool absorbing, float* tab) {
   double Q[6][6];
-  xb_generator<double>(prm, (double)pCa, (double)d, (double)centers[key % nb], (double)(key / nb),
-                       absorbing, Q);
-  const double dt = L.cfg.dt;
+  xb_generator<double>(prm, pCa, d, center, (double)(key / nb), absorbing, Q);
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -127,6 +108,50 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
     row[2] = (float)Q[3][0];
     row[3] = 0.0f;
   }
+}
+
+// All 2*nb XB (or absorbing) entries of one instance at spacing d, strided
+// over a workgroup.  Kept out of line: the FP64 6x6 expm needs far more
+// registers than the step kernel around it may hold.
+__device__ __attribute__((noinline)) void xb_table_rebuild(const float* prm, float pCa, double d, float dt,
+                                                           const float* centers, int nb, bool absorbing,
+                                                           float* tab, int tid, int nthreads) {
+  for (int key = tid; key < 2 * nb; key += nthreads)
+    xb_table_entry(prm, (double)pCa, (double)d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
+                   tab);
+}
+
+__global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_max) {
+  const int per = 27 + 4 * nb_max;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)(gid / per);
+  const int m = (int)(gid % per);
+  if (b >= L.B) return;
+  const mf_instance I = L.instances[b];
+  const mf_lattice G = L.lattices[I.lattice];
+  const int nb = G.n_bins;
+  const float* prm = L.params + (int64_t)b * MF_N_PARAMS;
+  const float* drv = L.drivers + ((int64_t)b * L.t_stride + I.t0 + L.t_base) * 3;
+  const float pCa = drv[1];
+  float d = drv[2];
+  if (I.flags & MF_INST_DYNAMIC) {
+    // dynamic spacing: the kinetics of the first step use the carried spacing
+    const StateLayout SL = state_layout(G.n_thick, G.n_crowns, G.n_thin, G.n_sites, G.n_xb_per_crown);
+    if (!(I.flags & MF_INST_INIT)) d = (float)*(const double*)(L.state + I.state_off + SL.ls);
+  }
+  float* tab = L.tables + I.table_off;
+  if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
+  const double dt = L.cfg.dt;
+  if (m < 27) {
+    tm_table_entry(prm, (double)pCa, dt, m, tab);
+    return;
+  }
+  int key = m - 27;
+  const bool absorbing = key >= 2 * nb_max;
+  if (absorbing) key -= 2 * nb_max;
+  if (key >= 2 * nb) return;
+  const float* centers = L.lat_f32 + G.bin_centers;
+  xb_table_entry(prm, (double)pCa, (double)d, dt, (double)centers[key % nb], key, nb, absorbing, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,8 +195,9 @@ enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, N
 enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF_SUM, F_X0 = NF_SUM, F_XT, NF };
 // Per-wave partial sums: pass 1 (node sums / maxima, XB expectations), pass 2,
 // and the pre-solve sum of thick positions (work = sum x_new - sum x_pre).
+// W_RG / W_RGD carry the radial sums of a dynamic-spacing residual.
 enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_ATPE, W_TEAR, W_SQK, W_SQN,
-                W_XPRE, NWP };
+                W_XPRE, W_RG, W_RGD, NWP };
 
 // All LDS arrays of one workgroup.
 template <bool REF64>
@@ -315,6 +341,21 @@ __device__ __forceinline__ void wg_sum2(Red<BS>& rd, float& a, float& b, int lan
   b = tb;
 }
 template <int BS>
+__device__ __forceinline__ void wg_sum3(Red<BS>& rd, float& a, float& b, float& c, int lane, int wave) {
+  LDS float* ra = rd.row();
+  LDS float* rb = rd.row();
+  LDS float* rc = rd.row();
+  const float sa = wave_sum(a, lane), sb = wave_sum(b, lane), sc = wave_sum(c, lane);
+  if (lane == 0) { ra[wave] = sa; rb[wave] = sb; rc[wave] = sc; }
+  __syncthreads();
+  float ta = 0.0f, tb = 0.0f, tc = 0.0f;
+#pragma unroll
+  for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; tc += rc[w]; }
+  a = ta;
+  b = tb;
+  c = tc;
+}
+template <int BS>
 __device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave) {
   LDS float* r = rd.row();
   const float s = wave_max(v, lane);
@@ -412,6 +453,42 @@ __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, flo
                T(sp.ck) * d * (d + T(2) * xd * (th - T(sp.cr))) / (r2 * r2));
 }
 
+// Spacing derivatives of a bound XB, for the bordered system of dynamic
+// lattice spacing (solver.py:552-656, forces.py:649-749): f_d = d f / d d,
+// the radial force g (forces.py:708-709) and g_d = d g / d d.  With
+// E the head's energy, f = dE/dxd and g = dE/dd, so d g / d xd = f_d.
+template <typename T>
+__device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, float& f_d, T& g, float& g_d) {
+  const T r = sqrt(xd * xd + d * d);
+  const T rs = r > T(1e-10) ? r : T(1e-10);
+  const T th = T(atan2f((float)d, (float)xd));
+  const T r3 = rs * rs * rs, r4 = r3 * rs;
+  const T dth = th - T(sp.cr);
+  f_d = (float)(T(sp.gk) * T(sp.gr) * xd * d / r3 - T(sp.ck) * (xd * d + dth * (xd * xd - d * d)) / r4);
+  g = T(sp.gk) * (rs - T(sp.gr)) * (d / rs) + (T(1) / rs) * T(sp.ck) * dth * (xd / rs);
+  g_d = (float)(T(sp.gk) * (T(1) - T(sp.gr) * xd * xd / r3) + T(sp.ck) * xd * (xd - T(2) * d * dth) / r4);
+}
+
+// Titin on the tip crown for the bordered system: the radial force tr
+// (N_TITIN F d / L), its spacing derivative tr_d, and tt_d = d(axial titin
+// force)/d d, which is also d(radial residual)/d x_tip.
+template <typename T>
+__device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T d, T& tr, float& tr_d,
+                                             float& tt_d) {
+  const T ad = z - tip;
+  const T Lq = sqrt(ad * ad + d * d);
+  const T Ls = Lq > T(1e-10) ? Lq : T(1e-10);
+  const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
+  const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
+  T F = T(prm[P_TITIN_A]) * exp(argc);
+  F = F > T(0) ? F : T(0);
+  const T bF = (arg > T(-100) && arg < T(100)) ? T(prm[P_TITIN_B]) * F : T(0);
+  tr = T(N_TITIN) * F * (d / Ls);
+  const T L2 = Ls * Ls, L3 = L2 * Ls;
+  tr_d = (float)(T(N_TITIN) * (bF * d * d / L2 + F * ad * ad / L3));
+  tt_d = (float)(T(N_TITIN) * ad * d * (bF / L2 - F / L3));
+}
+
 // Titin axial force on the tip crown and its derivative with respect to the
 // tip position (forces.py:136-157).
 template <typename T>
@@ -461,28 +538,55 @@ __device__ __forceinline__ void build_partners(const Shared<REF64>& sh, const Di
   }
 }
 
+// Dynamic lattice spacing: the spacing d is one more unknown, and the
+// Jacobian gains a border (solver.py:552-656).  All members are uniform
+// across the workgroup (every lane computes them from the same reductions).
+struct Radial {
+  bool on;      // this instance solves for d
+  float K;      // K_lat, already multiplied by n_thick (simulation.py:1002-1003)
+  float dref;   // d_ref = ls (z0 / z)^nu (simulation.py:694-696)
+  float frad;   // radial residual at the last residual evaluation
+  float jdd;    // d F_rad / d d at the last residual evaluation
+  float dinv;   // d-block preconditioner -1 / J_dd, taken at the solve's start
+};
+
 // Residual at the current positions: per-XB forces/stiffnesses (XB phase),
 // then per-node passive + XB sums (node phase), which also refreshes the
 // partners' stiffnesses.  Returns max|F| over the workgroup; f[] receives the
-// lane's slot residuals, ktip the tip stiffness.
-template <int BS, int MS, bool REF64, typename T>
+// lane's slot residuals, ktip the tip stiffness.  With dynamic spacing the
+// XB phase also forms d f / d d per head (kept in the CG-direction buffer,
+// idle here) and the radial sums; the node phase gathers the border column
+// cb = d F_axial / d d = d F_rad / d x, and the radial residual joins the max.
+template <int BS, int MS, bool REF64, bool DYN, typename T>
 __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
-                                          T (&f)[MS], float (&ktip)[MS], Partners<MS>& pt, Red<BS>& rd,
-                                          int tid, int lane, int wave) {
+                                          T (&f)[MS], float (&ktip)[MS], float (&cb)[MS], Partners<MS>& pt,
+                                          Radial& rad, Red<BS>& rd, int tid, int lane, int wave) {
   const LDS float* prm = sh.prm;
+  const bool dyn = DYN && rad.on;
+  LDS float* fdx = sh.pv;  // [X] d f / d d per head, dynamic spacing only
+  T sg = T(0), sgd = T(0);
   for (int j = tid; j < dm.X; j += BS) {
     const int site = sh.xsite[j];
     T fj = T(0);
-    float kj = 0.0f;
+    float kj = 0.0f, fd = 0.0f;
     const int st = sh.xst[j];
     if (site >= 0 && st >= 1 && st <= 3) {
       const int crown = j / dm.nx;
       const T xd = T(posd<REF64>(sh, site)) - T(posd<REF64>(sh, crown));
-      xb_force<T>(springs(prm, st), xd, d, fj, kj);
+      const Springs sp = springs(prm, st);
+      xb_force<T>(sp, xd, d, fj, kj);
+      if (dyn) {
+        T g;
+        float gd;
+        xb_radial<T>(sp, xd, d, fd, g, gd);
+        sg += g;
+        sgd += T(gd);
+      }
     }
     if constexpr (REF64) sh.fx64[j] = (double)fj; else sh.fx[j] = (float)fj;
     sh.kx[j] = kj;
+    if (dyn) fdx[j] = fd;
   }
   __syncthreads();
   const T kk = T(prm[P_THICK_K]), kn = T(prm[P_THIN_K]);
@@ -491,11 +595,13 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
   for (int s = 0; s < MS; ++s) {
     f[s] = T(0);
     ktip[s] = 0.0f;
+    cb[s] = 0.0f;
     const int n = sl.node[s];
     if (n < 0) continue;
     const int i = sl.li[s];
     const T x0 = T(posd<REF64>(sh, n));
     T xs = T(0);
+    float cs = 0.0f;
 #pragma unroll
     for (int h = 0; h < MAXP; ++h) {
       const int pk = pt.pk[s][h];
@@ -503,6 +609,7 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
       const int j = p_xb(pk);
       if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
       pt.k[s][h] = sh.kx[j];
+      if (dyn) cs += fdx[j];
     }
     T net;
     if (sl.thick[s]) {
@@ -513,9 +620,18 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
         s_hi = (T(posd<REF64>(sh, n + 1)) - x0 - T(rB[s])) * kk;
       } else {
         titin<T>(prm, x0, zl, d, s_hi, ktip[s]);
+        if (dyn) {
+          T tr;
+          float trd, ttd;
+          titin_radial<T>(prm, x0, zl, d, tr, trd, ttd);
+          sg += tr;
+          sgd += T(trd);
+          cs += ttd;
+        }
       }
       net = s_hi - s_lo;
       net += xs;
+      cb[s] = cs;
     } else {
       const T xp = i < dm.ns - 1 ? T(posd<REF64>(sh, n + 1)) : zl;
       const T s_hi = (xp - x0 - T(rA[s])) * kn;
@@ -526,14 +642,30 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
         for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
         for (; j >= 0; j = sh.xnext[j]) {
           if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+          if (dyn) cs += fdx[j];
         }
       }
       net -= xs;
+      cb[s] = -cs;
     }
     f[s] = net;
     // max|F| is taken in float: it is compared with the stopping tolerance
     // and reported as a float32 metric.
     mx = fmaxf(mx, fabsf((float)net));
+  }
+  if (dyn) {
+    // F_rad = -K (d - d_ref) - sum g - sum tr (solver.py:502-528);
+    // J_dd = -K - sum g_d - sum tr_d.  Summed in the residual's type, in a
+    // fixed wave order, so every lane gets the same bits.
+    const T a = wave_sum(sg, lane), c = wave_sum(sgd, lane);
+    if (lane == 0) { sh.wpart[wave * NWP + W_RG] = (double)a; sh.wpart[wave * NWP + W_RGD] = (double)c; }
+    __syncthreads();
+    T ga = T(0), gc = T(0);
+    for (int w = 0; w < BS / WAVE; ++w) { ga += T(sh.wpart[w * NWP + W_RG]); gc += T(sh.wpart[w * NWP + W_RGD]); }
+    const T fr = -T(rad.K) * (d - T(rad.dref)) - ga;
+    rad.frad = (float)fr;
+    rad.jdd = (float)(-T(rad.K) - gc);
+    mx = fmaxf(mx, fabsf(rad.frad));
   }
   return wg_max<BS>(rd, mx, lane, wave);
 }
@@ -586,28 +718,51 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
 // together, so each iteration costs one fused reduction and two barriers;
 // p and s = A p follow from u, w by the beta recurrence.  Refined mode only
 // needs the correction to converge: the FP64 residual decides when to stop.
-template <int BS, int MS, bool REF64>
+//
+// With dynamic spacing the system is bordered: the unknown d (uniform, held
+// by every lane) couples to the nodes through the column cb, so
+// A [v; v_d] = [-J v - cb v_d; -(cb . v) - J_dd v_d], and the d block of the
+// preconditioner is rad.dinv (solver.py:531-549).  cb . v joins the
+// iteration's existing reduction.
+template <int BS, int MS, bool REF64, bool DYN>
 __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
                                     const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
-                                    int n_cg, float (&x)[MS], Red<BS>& rd, int lane, int wave) {
+                                    const float (&cb)[MS], const Radial& rad, float b_d, int n_cg,
+                                    float (&x)[MS], float& x_d, Red<BS>& rd, int lane, int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
   const float ikk = 1.0f / kk, ikn = 1.0f / kn;
+  const bool dyn = DYN && rad.on;
+  x_d = 0.0f;
   if constexpr (REF64) {
     float r[MS], u[MS], w[MS], p[MS], q[MS];
 #pragma unroll
     for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; q[s] = 0.0f; }
+    float r_d = dyn ? b_d : 0.0f, p_d = 0.0f, q_d = 0.0f;
     float gamma = 0.0f, alpha = 1.0f;
     for (int it = 0;; ++it) {
       precond<MS>(r, u, sl, lane, ikk, ikn);
+      const float u_d = rad.dinv * r_d;
 #pragma unroll
       for (int s = 0; s < MS; ++s)
         if (sl.node[s] >= 0) sh.pv[sl.node[s]] = u[s];
       __syncthreads();
       neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, w);
-      float g = 0.0f, dl = 0.0f;
+      float g = 0.0f, dl = 0.0f, cu = 0.0f;
+      if (dyn) {
+#pragma unroll
+        for (int s = 0; s < MS; ++s) { w[s] -= cb[s] * u_d; cu += cb[s] * u[s]; }
+      }
 #pragma unroll
       for (int s = 0; s < MS; ++s) { g += r[s] * u[s]; dl += w[s] * u[s]; }
-      wg_sum2<BS>(rd, g, dl, lane, wave);
+      float w_d = 0.0f;
+      if (dyn) {
+        wg_sum3<BS>(rd, g, dl, cu, lane, wave);
+        w_d = -cu - rad.jdd * u_d;
+        g += r_d * u_d;
+        dl += w_d * u_d;
+      } else {
+        wg_sum2<BS>(rd, g, dl, lane, wave);
+      }
       const float beta = it == 0 ? 0.0f : g / (gamma + 1e-30f);
       const float den = it == 0 ? dl : dl - beta * g / alpha;
       alpha = g / (den + 1e-30f);
@@ -619,6 +774,12 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
         x[s] += alpha * p[s];
         r[s] -= alpha * q[s];
       }
+      if (dyn) {
+        p_d = u_d + beta * p_d;
+        q_d = w_d + beta * q_d;
+        x_d += alpha * p_d;
+        r_d -= alpha * q_d;
+      }
       if (it >= n_cg - 1) break;
     }
     return;
@@ -626,13 +787,16 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
   float r[MS], z[MS], p[MS], Ap[MS];
 #pragma unroll
   for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; }
+  float r_d = dyn ? b_d : 0.0f, p_d = 0.0f;
   float rz = 0.0f;
   for (int it = 0;; ++it) {
     precond<MS>(r, z, sl, lane, ikk, ikn);
+    const float z_d = rad.dinv * r_d;
     float loc = 0.0f;
 #pragma unroll
     for (int s = 0; s < MS; ++s) loc += r[s] * z[s];
-    const float rz_new = wg_sum<BS>(rd, loc, lane, wave);
+    float rz_new = wg_sum<BS>(rd, loc, lane, wave);
+    if (dyn) rz_new += r_d * z_d;
     const float beta = it == 0 ? 0.0f : rz_new / (rz + 1e-30f);
     rz = rz_new;
 #pragma unroll
@@ -640,30 +804,49 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
       p[s] = it == 0 ? z[s] : z[s] + beta * p[s];
       if (sl.node[s] >= 0) sh.pv[sl.node[s]] = p[s];
     }
+    p_d = it == 0 ? z_d : z_d + beta * p_d;
     __syncthreads();
     neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, Ap);
     loc = 0.0f;
+    float pAp, Ap_d = 0.0f;
+    if (dyn) {
+      float cp = 0.0f;
 #pragma unroll
-    for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
-    const float pAp = wg_sum<BS>(rd, loc, lane, wave);
+      for (int s = 0; s < MS; ++s) { Ap[s] -= cb[s] * p_d; cp += cb[s] * p[s]; loc += p[s] * Ap[s]; }
+      wg_sum2<BS>(rd, loc, cp, lane, wave);
+      Ap_d = -cp - rad.jdd * p_d;
+      pAp = loc + p_d * Ap_d;
+    } else {
+#pragma unroll
+      for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
+      pAp = wg_sum<BS>(rd, loc, lane, wave);
+    }
     const float alpha = rz / (pAp + 1e-30f);
 #pragma unroll
     for (int s = 0; s < MS; ++s) { x[s] += alpha * p[s]; r[s] -= alpha * Ap[s]; }
+    if (dyn) {
+      x_d += alpha * p_d;
+      r_d -= alpha * Ap_d;
+    }
     if (it >= n_cg - 1) break;
   }
 }
 
-// Equilibrium solve at spacing d, Z-line zl (solver.py:663-771, fixed spacing).
+// Equilibrium solve at Z-line zl (solver.py:663-771).  Fixed spacing: d is
+// the given spacing.  Dynamic spacing (rad.on): d is the initial guess and
+// returns the solved spacing, clamped at >= 1 nm after every Newton step
+// (solver.py:599-656).
 // Reference mode: FP32 residual, stop at max|F| <= tol or n_newton.
 // Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or n_refine.
-template <int BS, int MS, bool REF64>
+template <int BS, int MS, bool REF64, bool DYN>
 __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
-                                      const float (&rA)[MS], const float (&rB)[MS], float zl, float d,
-                                      const mf_solver_cfg& cfg, Red<BS>& rd, int tid, int lane, int wave,
-                                      int& iters, float& resid) {
+                                      const float (&rA)[MS], const float (&rB)[MS], float zl,
+                                      typename std::conditional<REF64, double, float>::type& d,
+                                      const mf_solver_cfg& cfg, Radial& rad, Red<BS>& rd, int tid, int lane,
+                                      int wave, int& iters, float& resid) {
   using T = typename std::conditional<REF64, double, float>::type;
   T f[MS];
-  float ktip[MS], dx[MS], fb[MS];
+  float ktip[MS], dx[MS], fb[MS], cb[MS];
   float tol;
   int cap;
   if constexpr (REF64) {
@@ -679,17 +862,25 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
   int it = 0;
   float mx;
   for (;;) {
-    mx = residual<BS, MS, REF64, T>(sh, dm, sl, rA, rB, T(zl), T(d), f, ktip, pt, rd, tid, lane, wave);
+    mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, f, ktip, cb, pt, rad, rd, tid, lane,
+                                         wave);
+    // the d-block preconditioner uses J_dd at the initial state (solver.py:633-638)
+    if (DYN && rad.on && it == 0) rad.dinv = -1.0f / rad.jdd;
     if (!(mx > tol && it < cap)) break;
 #pragma unroll
     for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
-    pcg<BS, MS, REF64>(sh, dm, sl, pt, fb, ktip, cfg.n_cg, dx, rd, lane, wave);
+    float dd;
+    pcg<BS, MS, REF64, DYN>(sh, dm, sl, pt, fb, ktip, cb, rad, rad.frad, cfg.n_cg, dx, dd, rd, lane, wave);
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
       const int n = sl.node[s];
       if (n < 0) continue;
       const float v = isfinite(dx[s]) ? dx[s] : 0.0f;
       if constexpr (REF64) sh.x64[n] += (double)v; else sh.x32[n] += v;
+    }
+    if (DYN && rad.on) {
+      d += T(isfinite(dd) ? dd : 0.0f);
+      d = d > T(1) ? d : T(1);
     }
     __syncthreads();
     ++it;
@@ -749,7 +940,7 @@ __device__ __forceinline__ void wave_count(LDS int* c, bool pred, int lane) {
   if (lane == 0 && m) atomicAdd((int*)c, (int)__popcll(m));
 }
 
-template <int BS, int MS, int MX, bool REF64>
+template <int BS, int MS, int MX, bool REF64, bool DYN>
 __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   extern __shared__ __align__(16) char smem[];
   constexpr int NW = BS / WAVE;
@@ -879,6 +1070,17 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   for (int s = tid; s < S; s += BS) sh.shead[s] = -1;
   __syncthreads();
 
+  // dynamic lattice spacing (DYN variants, MF_INST_DYNAMIC instances)
+  Radial rad;
+  rad.on = DYN && (I.flags & MF_INST_DYNAMIC) != 0;
+  rad.K = I.K_lat;
+  rad.dref = 0.0f;
+  rad.frad = 0.0f;
+  rad.jdd = -1.0f;
+  rad.dinv = 0.0f;
+  const float z_trace0 = drv[0];  // l0 of the Poisson reference spacing (simulation.py:676)
+  const float* centers = lf + G.bin_centers;
+
   const uint64_t seed = I.seed;
   const int n_steps = I.n_steps, t_first = I.t0 + L.t_base;
   const float ov_lo = G.overlap_lo, ov_hi = G.overlap_hi;
@@ -895,6 +1097,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     const int t = t_first + (kin ? step : 0);
     const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
     float z_solve, d_solve;
+    // old key and state of each head, for the dynamic-spacing metrics (their
+    // absorbing chain is evaluated at the solved spacing)
+    int q_kold[MX], q_sold[MX];
     if (!kin) {
       // Nearest update, then a solve against the parameter row's z_line and
       // lattice_spacing (the reference's run() leaves those at their defaults).
@@ -911,6 +1116,13 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
       z_solve = zl;
       d_solve = lsd;
+      if (DYN && rad.on) {
+        // kinetics at the carried spacing (simulation.py:691-693): rebuild
+        // the XB sampling rows; d_ref = ls (z0 / z)^nu (simulation.py:694-696)
+        xb_table_rebuild(prm_g, pCa, ls_carry, L.cfg.dt, centers, nb, false, tab, tid, BS);
+        __syncthreads();
+        rad.dref = lsd * powf(z_trace0 / zl, I.nu);
+      }
       // thin.axial += dz (simulation.py:683-687)
       if (dz != 0.0f) {
         for (int i = dm.nk + tid; i < dm.N; i += BS) {
@@ -967,7 +1179,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const float* r1 = xbtab + (int64_t)(nb + bn) * 48 + old * 8;
           q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
           q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
-          if (old == 2 || old == 3) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
+          if (!(DYN && rad.on) && (old == 2 || old == 3)) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
+          if (DYN) { q_kold[q] = keyold; q_sold[q] = old; }
           q_bin[q] = bn;
           q_occ[q] = sh.tbound[q_near[q]] >= 0;
           q_old[q] = old;
@@ -1028,7 +1241,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           sh.xkey[j] = (short)key;
           if (site >= 0) sh.xnext[j] = (short)atomicExch((int*)(sh.shead + (site - dm.nk)), j);
           // expected ATP / tear from the previous step's key (metrics_fn.py:297-335)
-          if (old == 2 || old == 3) {
+          if (!(DYN && rad.on) && (old == 2 || old == 3)) {
             const float4 a = q_ab[q];
             if (old == 3) m_atpe += a.x;
             m_tear += old == 3 ? a.z : a.y;
@@ -1050,10 +1263,34 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     // ---- equilibrium solve -------------------------------------------------
     int iters;
     float resid;
-    solve<BS, MS, REF64>(sh, dm, sl, rA, rB, z_solve, d_solve, L.cfg, rd, tid, lane, wave, iters, resid);
+    using TS = typename std::conditional<REF64, double, float>::type;
+    const bool rad_on = rad.on;
+    rad.on = rad_on && kin;  // the initial equilibration runs at fixed spacing
+    TS d_sol = rad.on ? TS(ls_carry) : TS(d_solve);
+    solve<BS, MS, REF64, DYN>(sh, dm, sl, rA, rB, z_solve, d_sol, L.cfg, rad, rd, tid, lane, wave, iters,
+                              resid);
+    rad.on = rad_on;
     if (!kin) continue;
-    const double ls_new = (double)d_solve;
+    const double ls_new = (DYN && rad.on) ? (double)d_sol : (double)d_solve;
     ls_carry = ls_new;
+    if (DYN && rad.on) {
+      // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
+      // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
+      xb_table_rebuild(prm_g, pCa, ls_new, L.cfg.dt, centers, nb, true, tab, tid, BS);
+      __syncthreads();
+      float m_atpe = 0.0f, m_tear = 0.0f;
+#pragma unroll
+      for (int q = 0; q < MX; ++q) {
+        const int j = tid + q * BS;
+        if (j < X && (q_sold[q] == 2 || q_sold[q] == 3)) {
+          const float4 a = *(const float4*)(abstab + (int64_t)q_kold[q] * 4);
+          if (q_sold[q] == 3) m_atpe += a.x;
+          m_tear += q_sold[q] == 3 ? a.z : a.y;
+        }
+      }
+      const float ae = wave_sum(m_atpe, lane), te = wave_sum(m_tear, lane);
+      if (lane == 0) { sh.wpart[wave * NWP + W_ATPE] = ae; sh.wpart[wave * NWP + W_TEAR] = te; }
+    }
 
     // ---- metrics (metrics_fn.py:154-417) -----------------------------------
     if (L.metrics != nullptr || L.force != nullptr) {
@@ -1228,20 +1465,26 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
 // ---------------------------------------------------------------------------
 struct Variant {
   int bs, ms, mx;
-  bool ref64;
+  bool ref64, dyn;
   const void* fn;
   int64_t (*lds)(const Dims&);
 };
 
-template <int BS, int MS, int MX, bool R>
+template <int BS, int MS, int MX, bool R, bool D = false>
 Variant make_variant() {
-  return Variant{BS, MS, MX, R, (const void*)&run_kernel<BS, MS, MX, R>, &lds_bytes<BS, R>};
+  return Variant{BS, MS, MX, R, D, (const void*)&run_kernel<BS, MS, MX, R, D>, &lds_bytes<BS, R>};
 }
 
+// Fixed-spacing variants first; the dynamic-spacing (bordered solve) ones
+// also run fixed-spacing instances, and are picked only for launches that
+// carry a dynamic instance.
 const Variant* variants(int& n) {
   static const Variant v[] = {
       make_variant<1024, 2, 1, false>(), make_variant<1024, 2, 1, true>(),
       make_variant<768, 2, 1, false>(),  make_variant<768, 2, 1, true>(),
+      make_variant<768, 2, 1, false, true>(),  make_variant<768, 2, 1, true, true>(),
+      make_variant<1024, 2, 1, false, true>(), make_variant<1024, 2, 1, true, true>(),
+      make_variant<1024, 8, 4, false, true>(), make_variant<1024, 8, 4, true, true>(),
       make_variant<1024, 8, 4, false>(), make_variant<1024, 8, 4, true>(),
       make_variant<512, 4, 2, false>(),  make_variant<512, 4, 2, true>(),
       make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
@@ -1277,8 +1520,9 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
   int n;
   const Variant* v = variants(n);
   const bool r64 = L->cfg.mode == MF_SOLVER_REFINED;
+  const bool dyn = (L->launch_flags & MF_LAUNCH_DYNAMIC) != 0;
   for (int i = 0; i < n; ++i) {
-    if (v[i].ref64 != r64 || v[i].bs != L->block) continue;
+    if (v[i].ref64 != r64 || v[i].dyn != dyn || v[i].bs != L->block) continue;
     bool ok = true;
     int64_t need = 0;
     for (const auto& G : lats) {
@@ -1287,6 +1531,8 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
       needs(d, v[i].bs, ms, mx);
       int nchw = (d.nch + v[i].bs / WAVE - 1) / (v[i].bs / WAVE);
       if (ms > v[i].ms || mx > v[i].mx || nchw > 8 || d.N > 32767) ok = false;
+      // the bordered solve keeps d f / d d per head in the [N] CG buffer
+      if (v[i].dyn && d.X > d.N) ok = false;
       need = std::max(need, v[i].lds(d));
     }
     if (ok && need <= 160 * 1024) { *out = &v[i]; *lds = need; return 0; }

@@ -34,7 +34,7 @@ def _al8(v: int) -> int:
 
 def state_layout(topo) -> dict:
     """Byte offsets of an instance state record (multifil_amd.h)."""
-    N, X, S = topo.n_dof, topo.total_xbs, topo.n_thin * topo.n_sites
+    N, X, S = topo.n_thick * topo.n_crowns + topo.n_thin * topo.n_sites, topo.total_xbs, topo.n_thin * topo.n_sites
     o = {"pos": 0}
     o["ls"] = 8 * N
     o["xb_bound"] = o["ls"] + 8
@@ -159,11 +159,16 @@ class Batch:
     params: (B, 44) float32 parameter rows.
     drivers: (B, T, 3) float32 (z_line, pCa, lattice_spacing per step).
     seeds: (B,) uint64 Philox keys.
-    dynamic: None, or (K_lat (B,), nu (B,)) for dynamic lattice spacing.
+    dynamic: None, or (K_lat (B,), nu (B,)) for dynamic lattice spacing;
+        K_lat is per instance and already multiplied by n_thick (as run()
+        does, simulation.py:1002-1003); an instance with K_lat = NaN keeps a
+        fixed spacing.
+    lengths: optional (B,) steps per instance (<= T); an instance stops
+        after its own length, its later metric rows stay zero.
     """
 
     def __init__(self, pack: LatticePack, lattice_ids, params, drivers, seeds,
-                 dynamic=None, device: str = "cuda", want_metrics: bool = True):
+                 dynamic=None, device: str = "cuda", want_metrics: bool = True, lengths=None):
         torch = _torch()
         lib = _lib.load()
         self.lib = lib
@@ -181,9 +186,17 @@ class Batch:
         self.params_np = params
         self.drivers_np = drivers
         self.seeds = np.asarray(seeds, dtype=np.uint64)
-        self.dynamic = dynamic
+        self.K_lat = np.full(self.B, np.nan, np.float32)
+        self.nu = np.zeros(self.B, np.float32)
         if dynamic is not None:
-            raise NotImplementedError("dynamic lattice spacing is not available in this engine build")
+            K, nu = dynamic
+            self.K_lat[:] = np.broadcast_to(np.asarray(K, np.float32), (self.B,))
+            self.nu[:] = np.broadcast_to(np.asarray(nu, np.float32), (self.B,))
+        self.is_dynamic = np.isfinite(self.K_lat)
+        self.lengths = (np.full(self.B, self.T, np.int64) if lengths is None
+                        else np.asarray(lengths, np.int64))
+        if self.lengths.shape != (self.B,) or np.any(self.lengths < 0) or np.any(self.lengths > self.T):
+            raise ValueError(f"lengths must be ({self.B},) in [0, {self.T}]")
 
         topos = pack.topos
         self.layouts = [state_layout(topos[i]) for i in self.lattice_ids]
@@ -216,18 +229,22 @@ class Batch:
         self._table_dt = None
 
     # ------------------------------------------------------------------
-    def _instances(self, n_steps: int, init_mask):
-        """Device instance table for a launch (cached: per-launch step
-        offsets travel in mf_launch.t_base)."""
-        key = (int(n_steps), init_mask.tobytes())
+    def _instances(self, n_steps: int, init_mask, s0: int = 0):
+        """Device instance table for a launch over steps [s0, s0 + n_steps)
+        (cached; the step offset itself travels in mf_launch.t_base)."""
+        steps = np.clip(self.lengths - s0, 0, n_steps)
+        key = (int(n_steps), init_mask.tobytes(), steps.tobytes())
         hit = self._inst_cache.get(key)
         if hit is None:
             arr = (_lib.MfInstance * self.B)()
             for b in range(self.B):
                 arr[b].lattice = int(self.lattice_ids[b])
-                arr[b].n_steps = int(n_steps)
+                arr[b].n_steps = int(steps[b])
                 arr[b].t0 = 0
-                arr[b].flags = _lib.MF_INST_INIT if init_mask[b] else 0
+                arr[b].flags = (_lib.MF_INST_INIT if init_mask[b] else 0) | \
+                    (_lib.MF_INST_DYNAMIC if self.is_dynamic[b] else 0)
+                arr[b].K_lat = float(self.K_lat[b]) if self.is_dynamic[b] else 0.0
+                arr[b].nu = float(self.nu[b])
                 arr[b].seed = int(self.seeds[b])
                 arr[b].state_off = int(self.state_off[b])
                 arr[b].table_off = int(self.table_off[b])
@@ -255,6 +272,7 @@ class Batch:
         L.u_stride = int(u_stride)
         L.cfg = cfg.ctype()
         L.block = int(self.block)
+        L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if self.is_dynamic.any() else 0
         L.lattices_host = C.addressof(self.pack.desc_host)
         return L
 
@@ -299,7 +317,7 @@ class Batch:
         segs = self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]
         for (s0, n) in segs:
             init = ~self.initialized
-            self._instances(n, init)
+            self._instances(n, init, s0)
             L = self._launch(cfg, uniforms, u_stride)
             L.t_base = int(s0)
             rows = self.drivers_np[:, s0, 1:3]
@@ -323,7 +341,7 @@ class Batch:
         t = self.pack.topos[self.lattice_ids[b]]
         lay = self.layouts[b]
         raw = self.d_state[self.state_off[b]:self.state_off[b] + lay["bytes"]].cpu().numpy()
-        N, X, S = t.n_dof, t.total_xbs, t.n_thin * t.n_sites
+        N, X, S = t.n_thick * t.n_crowns + t.n_thin * t.n_sites, t.total_xbs, t.n_thin * t.n_sites
         return {
             "pos": raw[lay["pos"]:lay["pos"] + 8 * N].view(np.float64).copy(),
             "ls": float(raw[lay["ls"]:lay["ls"] + 8].view(np.float64)[0]),
@@ -340,7 +358,7 @@ class Batch:
         t = self.pack.topos[self.lattice_ids[b]]
         lay = self.layouts[b]
         raw = np.zeros(lay["bytes"], np.uint8)
-        N, X, S = t.n_dof, t.total_xbs, t.n_thin * t.n_sites
+        N, X, S = t.n_thick * t.n_crowns + t.n_thin * t.n_sites, t.total_xbs, t.n_thin * t.n_sites
 
         def put(name, arr, dtype, n):
             a = np.ascontiguousarray(arr, dtype=dtype).ravel()
