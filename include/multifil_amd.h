@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 2
+#define MF_ABI_VERSION 3
 #define MF_N_PARAMS 44      /* DynamicParams fields, params.py:73-553 order */
 #define MF_N_METRICS 52     /* metrics_fn.py:343-417 order                */
 #define MF_MAX_BINS 256     /* n_xb_bins upper bound                       */
@@ -124,6 +124,10 @@ typedef struct mf_launch {
   int32_t pad;
   const mf_lattice* lattices_host; /* optional HOST copy of `lattices`; when
                                    NULL the library reads the device copy  */
+  uint8_t* workspace;           /* [B][workspace_stride] device scratch for
+                                   lattices whose per-instance arrays exceed
+                                   LDS; NULL when mf_workspace_bytes gives 0 */
+  int64_t workspace_stride;     /* bytes per instance, >= mf_workspace_bytes */
 } mf_launch;
 
 /* Byte layout of one instance state (all sections 8-byte aligned):
@@ -148,6 +152,11 @@ int mf_tables(const mf_launch* L, void* stream);
 
 /* Advance every instance n_steps (one persistent workgroup per instance). */
 int mf_run(const mf_launch* L, void* stream);
+
+/* Device workspace bytes per instance that mf_run needs for this launch's
+ * lattices, block and mode (0: everything fits in LDS).  No counterpart in
+ * the reference, which leaves all buffers to XLA. */
+int mf_workspace_bytes(const mf_launch* launch, int64_t* per_instance);
 
 /* Dynamic LDS bytes and the workgroup size mf_run would use (diagnostics). */
 int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes);

@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmultifil_amd.so")
 
-MF_ABI_VERSION = 2
+MF_ABI_VERSION = 3
 MF_N_PARAMS = 44
 MF_N_METRICS = 52
 MF_MAX_BINS = 256
@@ -23,7 +23,7 @@ MF_INST_DYNAMIC = 2
 MF_LAUNCH_DYNAMIC = 1
 
 EXPORTED = ("mf_abi_version", "mf_last_error", "mf_state_bytes", "mf_table_floats",
-            "mf_tables", "mf_run", "mf_run_lds_bytes")
+            "mf_tables", "mf_run", "mf_run_lds_bytes", "mf_workspace_bytes")
 
 
 class MfLattice(C.Structure):
@@ -57,7 +57,8 @@ class MfLaunch(C.Structure):
                 ("n_lattices", C.c_int32), ("B", C.c_int32), ("t_stride", C.c_int32),
                 ("u_stride", C.c_int32), ("cfg", MfSolverCfg), ("block", C.c_int32),
                 ("t_base", C.c_int32), ("launch_flags", C.c_int32), ("pad", C.c_int32),
-                ("lattices_host", C.c_void_p)]
+                ("lattices_host", C.c_void_p), ("workspace", C.c_void_p),
+                ("workspace_stride", C.c_int64)]
 
 
 _lib = None
@@ -86,8 +87,9 @@ def load() -> C.CDLL:
     for fn in ("mf_tables", "mf_run"):
         getattr(lib, fn).restype = C.c_int
         getattr(lib, fn).argtypes = [C.POINTER(MfLaunch), C.c_void_p]
-    lib.mf_run_lds_bytes.restype = C.c_int
-    lib.mf_run_lds_bytes.argtypes = [C.POINTER(MfLaunch), C.POINTER(C.c_int64)]
+    for fn in ("mf_run_lds_bytes", "mf_workspace_bytes"):
+        getattr(lib, fn).restype = C.c_int
+        getattr(lib, fn).argtypes = [C.POINTER(MfLaunch), C.POINTER(C.c_int64)]
     if lib.mf_abi_version() != MF_ABI_VERSION:
         raise RuntimeError(f"ABI mismatch: library {lib.mf_abi_version()} != {MF_ABI_VERSION}")
     _lib = lib

@@ -199,28 +199,37 @@ enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF_SUM, F_X0 = NF_SUM, F_
 enum WavePart { W_SUMK = 0, W_SUMN, W_WORK, W_MAXK, W_NMINK, W_MAXN, W_NMINN, W_ATPE, W_TEAR, W_SQK, W_SQN,
                 W_XPRE, W_RG, W_RGD, NWP };
 
-// All LDS arrays of one workgroup.
-template <bool REF64>
+// All per-instance arrays of one workgroup.  Positions, the arrays updated
+// by atomics (site lists, thin.bound_to) and the small tables always sit in
+// LDS.  The other per-node / per-XB / per-site arrays ("Big") sit in LDS
+// too, except in the global-workspace (GW) variants for lattices whose
+// arrays exceed the 160 KB of LDS: there they live in the instance's slice
+// of a device workspace.  Only the waves of this workgroup touch that slice,
+// with plain loads and stores, and they share one CU and its write-through
+// L1, so a workgroup barrier orders those accesses as it orders LDS ones
+// (atomics, which execute in L2, are kept to LDS arrays).
+template <bool REF64, bool GW = false>
 struct Shared {
+  template <class T> using Big = typename std::conditional<GW, T*, LDS T*>::type;
   LDS float* x32;        // [N]   positions (reference mode)
   LDS double* x64;       // [N]   positions (refined mode)
-  LDS float* pv;         // [N]   CG direction
-  LDS float* fx;         // [X]   XB axial force (reference mode)
-  LDS double* fx64;      // [X]   XB axial force (refined mode)
-  LDS float* kx;         // [X]   XB axial stiffness df/dx
+  Big<float> pv;         // [N]   CG direction
+  Big<float> fx;         // [X]   XB axial force (reference mode)
+  Big<double> fx64;      // [X]   XB axial force (refined mode)
+  Big<float> kx;         // [X]   XB axial stiffness df/dx
   LDS short* xsite;      // [X]   DOF index of the bound site, -1 unbound
-  LDS short* xnext;      // [X]   next XB in the site's bound list
-  LDS short* xkey;       // [X]   table row of the last kinetics step (< 2 * MF_MAX_BINS)
-  LDS short* xslot;      // [X]   (thin, face) slot | target-valid << 15
-  LDS float* org;        // [N]   displacement origin: crown offset / cumulative thin rest
+  Big<short> xnext;      // [X]   next XB in the site's bound list
+  Big<short> xkey;       // [X]   table row of the last kinetics step (< 2 * MF_MAX_BINS)
+  Big<short> xslot;      // [X]   (thin, face) slot | target-valid << 15
+  Big<float> org;        // [N]   displacement origin: crown offset / cumulative thin rest
   LDS unsigned char* xst;  // [X] XB state
   LDS int* shead;        // [S]   first XB bound to the site, -1 none
   LDS int* tbound;       // [S]   thin.bound_to
   LDS unsigned char* tm; // [2][S] TM state (double buffered)
-  LDS short* fsites;     // [Tn*nf*W] candidate sites per (thin, face)
-  LDS short* fcount;     // [Tn*nf]
-  LDS signed char* tprev;  // [S] same-chain neighbours as offsets along the filament
-  LDS signed char* tnext;  // [S] (0: none, the site references itself)
+  Big<short> fsites;     // [Tn*nf*W] candidate sites per (thin, face)
+  Big<short> fcount;     // [Tn*nf]
+  Big<signed char> tprev;  // [S] same-chain neighbours as offsets along the filament
+  Big<signed char> tnext;  // [S] (0: none, the site references itself)
   LDS float* tmcum;      // [27*16]
   LDS float* edges;      // [nb+1]
   LDS float* prm;        // [MF_N_PARAMS]
@@ -230,35 +239,43 @@ struct Shared {
   LDS int* cnt;          // [NC] metric counters
 };
 
-// LDS carve-up.  One routine computes the byte offsets (host: the dynamic
-// LDS size; device: the typed pointers), so the two cannot disagree.
+// LDS / workspace carve-up.  One routine computes the byte offsets (host:
+// the dynamic LDS size and the workspace slice; device: the typed
+// pointers), so the two cannot disagree.  Offsets of "Big" arrays are into
+// the workspace slice when GW is set.
 struct LdsOffsets {
   int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
-      edges, prm, red, wpart, fil, cnt, total;
+      edges, prm, red, wpart, fil, cnt, total, ws_total;
 };
-template <int BS, bool REF64>
+template <int BS, bool REF64, bool GW = false>
 __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   constexpr int NW = BS / WAVE;
   LdsOffsets L;
-  int64_t o = 0;
+  int64_t o = 0, w = 0;
   auto take = [&](int64_t bytes) { const int64_t at = o; o = (o + bytes + 15) & ~int64_t(15); return at; };
+  auto big = [&](int64_t bytes) {
+    if (!GW) return take(bytes);
+    const int64_t at = w;
+    w = (w + bytes + 255) & ~int64_t(255);
+    return at;
+  };
   L.x = take((REF64 ? 8LL : 4LL) * d.N);
-  L.pv = take(4LL * d.N);
-  L.fx = take((REF64 ? 8LL : 4LL) * d.X);
-  L.kx = take(4LL * d.X);
+  L.pv = big(4LL * d.N);
+  L.fx = big((REF64 ? 8LL : 4LL) * d.X);
+  L.kx = big(4LL * d.X);
   L.xsite = take(2LL * d.X);
-  L.xnext = take(2LL * d.X);
-  L.xkey = take(2LL * d.X);
-  L.xslot = take(2LL * d.X);
-  L.org = take(4LL * d.N);
+  L.xnext = big(2LL * d.X);
+  L.xkey = big(2LL * d.X);
+  L.xslot = big(2LL * d.X);
+  L.org = big(4LL * d.N);
   L.xst = take(d.X);
   L.shead = take(4LL * d.S);
   L.tbound = take(4LL * d.S);
   L.tm = take(2LL * d.S);
-  L.fsites = take(2LL * d.Tn * d.nf * d.W);
-  L.fcount = take(2LL * d.Tn * d.nf);
-  L.tprev = take(d.S);
-  L.tnext = take(d.S);
+  L.fsites = big(2LL * d.Tn * d.nf * d.W);
+  L.fcount = big(2LL * d.Tn * d.nf);
+  L.tprev = big(d.S);
+  L.tnext = big(d.S);
   L.tmcum = take(4 * 27 * 16);
   L.edges = take(4LL * (d.nb + 1));
   L.prm = take(4 * MF_N_PARAMS);
@@ -267,34 +284,42 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.fil = take(8LL * d.Tk * NF);
   L.cnt = take(4 * NC);
   L.total = o;
+  L.ws_total = w;
   return L;
 }
-template <int BS, bool REF64>
-int64_t lds_bytes(const Dims& d) { return lds_offsets<BS, REF64>(d).total; }
+template <int BS, bool REF64, bool GW>
+int64_t lds_bytes(const Dims& d) { return lds_offsets<BS, REF64, GW>(d).total; }
+template <int BS, bool REF64, bool GW>
+int64_t ws_bytes(const Dims& d) { return lds_offsets<BS, REF64, GW>(d).ws_total; }
 
-template <int BS, bool REF64>
-__device__ __forceinline__ Shared<REF64> carve(LDS char* base, const Dims& d) {
-  const LdsOffsets o = lds_offsets<BS, REF64>(d);
-  Shared<REF64> t;
+template <int BS, bool REF64, bool GW>
+__device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, const Dims& d) {
+  const LdsOffsets o = lds_offsets<BS, REF64, GW>(d);
+  using S = Shared<REF64, GW>;
+  // a Big array's base: the workspace slice in GW variants, LDS otherwise
+  auto bb = [&](int64_t off) {
+    if constexpr (GW) return (char*)(ws + off); else return (LDS char*)(base + off);
+  };
+  S t;
   t.x32 = (LDS float*)(base + o.x);
   t.x64 = (LDS double*)(base + o.x);
-  t.pv = (LDS float*)(base + o.pv);
-  t.fx = (LDS float*)(base + o.fx);
-  t.fx64 = (LDS double*)(base + o.fx);
-  t.kx = (LDS float*)(base + o.kx);
+  t.pv = (typename S::template Big<float>)bb(o.pv);
+  t.fx = (typename S::template Big<float>)bb(o.fx);
+  t.fx64 = (typename S::template Big<double>)bb(o.fx);
+  t.kx = (typename S::template Big<float>)bb(o.kx);
   t.xsite = (LDS short*)(base + o.xsite);
-  t.xnext = (LDS short*)(base + o.xnext);
-  t.xkey = (LDS short*)(base + o.xkey);
-  t.xslot = (LDS short*)(base + o.xslot);
-  t.org = (LDS float*)(base + o.org);
+  t.xnext = (typename S::template Big<short>)bb(o.xnext);
+  t.xkey = (typename S::template Big<short>)bb(o.xkey);
+  t.xslot = (typename S::template Big<short>)bb(o.xslot);
+  t.org = (typename S::template Big<float>)bb(o.org);
   t.xst = (LDS unsigned char*)(base + o.xst);
   t.shead = (LDS int*)(base + o.shead);
   t.tbound = (LDS int*)(base + o.tbound);
   t.tm = (LDS unsigned char*)(base + o.tm);
-  t.fsites = (LDS short*)(base + o.fsites);
-  t.fcount = (LDS short*)(base + o.fcount);
-  t.tprev = (LDS signed char*)(base + o.tprev);
-  t.tnext = (LDS signed char*)(base + o.tnext);
+  t.fsites = (typename S::template Big<short>)bb(o.fsites);
+  t.fcount = (typename S::template Big<short>)bb(o.fcount);
+  t.tprev = (typename S::template Big<signed char>)bb(o.tprev);
+  t.tnext = (typename S::template Big<signed char>)bb(o.tnext);
   t.tmcum = (LDS float*)(base + o.tmcum);
   t.edges = (LDS float*)(base + o.edges);
   t.prm = (LDS float*)(base + o.prm);
@@ -415,12 +440,12 @@ __device__ __forceinline__ void precond(const float (&r)[MS], float (&z)[MS], co
   for (int s = 0; s < MS; ++s) z[s] *= sl.thick[s] ? inv_kk : inv_kn;
 }
 
-template <bool REF64>
-__device__ __forceinline__ double posd(const Shared<REF64>& sh, int i) {
+template <bool REF64, bool GW>
+__device__ __forceinline__ double posd(const Shared<REF64, GW>& sh, int i) {
   if constexpr (REF64) return sh.x64[i]; else return (double)sh.x32[i];
 }
-template <bool REF64>
-__device__ __forceinline__ float posf(const Shared<REF64>& sh, int i) {
+template <bool REF64, bool GW>
+__device__ __forceinline__ float posf(const Shared<REF64, GW>& sh, int i) {
   if constexpr (REF64) return (float)sh.x64[i]; else return sh.x32[i];
 }
 
@@ -506,8 +531,8 @@ __device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& 
 }
 
 // Partners of the slot nodes for the current bindings (start of a solve).
-template <int MS, bool REF64>
-__device__ __forceinline__ void build_partners(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+template <int MS, bool REF64, bool GW>
+__device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                                Partners<MS>& pt) {
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
@@ -557,14 +582,14 @@ struct Radial {
 // XB phase also forms d f / d d per head (kept in the CG-direction buffer,
 // idle here) and the radial sums; the node phase gathers the border column
 // cb = d F_axial / d d = d F_rad / d x, and the radial residual joins the max.
-template <int BS, int MS, bool REF64, bool DYN, typename T>
-__device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+template <int BS, int MS, bool REF64, bool DYN, typename T, bool GW>
+__device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
                                           T (&f)[MS], float (&ktip)[MS], float (&cb)[MS], Partners<MS>& pt,
                                           Radial& rad, Red<BS>& rd, int tid, int lane, int wave) {
   const LDS float* prm = sh.prm;
   const bool dyn = DYN && rad.on;
-  LDS float* fdx = sh.pv;  // [X] d f / d d per head, dynamic spacing only
+  const auto fdx = sh.pv;  // [X] d f / d d per head, dynamic spacing only
   T sg = T(0), sgd = T(0);
   for (int j = tid; j < dm.X; j += BS) {
     const int site = sh.xsite[j];
@@ -671,8 +696,8 @@ __device__ __forceinline__ float residual(const Shared<REF64>& sh, const Dims& d
 }
 
 // -J p for the slot nodes (p read from LDS).
-template <int MS, bool REF64>
-__device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+template <int MS, bool REF64, bool GW>
+__device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                         const Partners<MS>& pt, const float (&ktip)[MS], float kk, float kn,
                                         float (&Ap)[MS]) {
 #pragma unroll
@@ -724,8 +749,8 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64>& sh, const Dims& dm,
 // A [v; v_d] = [-J v - cb v_d; -(cb . v) - J_dd v_d], and the d block of the
 // preconditioner is rad.dinv (solver.py:531-549).  cb . v joins the
 // iteration's existing reduction.
-template <int BS, int MS, bool REF64, bool DYN>
-__device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+template <int BS, int MS, bool REF64, bool DYN, bool GW>
+__device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                     const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
                                     const float (&cb)[MS], const Radial& rad, float b_d, int n_cg,
                                     float (&x)[MS], float& x_d, Red<BS>& rd, int lane, int wave) {
@@ -838,8 +863,8 @@ __device__ __forceinline__ void pcg(const Shared<REF64>& sh, const Dims& dm, con
 // (solver.py:599-656).
 // Reference mode: FP32 residual, stop at max|F| <= tol or n_newton.
 // Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or n_refine.
-template <int BS, int MS, bool REF64, bool DYN>
-__device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, const Slots<MS>& sl,
+template <int BS, int MS, bool REF64, bool DYN, bool GW>
+__device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                       const float (&rA)[MS], const float (&rB)[MS], float zl,
                                       typename std::conditional<REF64, double, float>::type& d,
                                       const mf_solver_cfg& cfg, Radial& rad, Red<BS>& rd, int tid, int lane,
@@ -895,13 +920,13 @@ __device__ __forceinline__ void solve(const Shared<REF64>& sh, const Dims& dm, c
 // Candidates are gathered in chunks of NEAR_CHUNK so a chunk's LDS loads are
 // in flight together without holding all W positions in registers.
 constexpr int NEAR_CHUNK = 8;
-template <bool REF64>
-__device__ __forceinline__ void nearest_site(const Shared<REF64>& sh, const Dims& dm, int slot, int thin,
+template <bool REF64, bool GW>
+__device__ __forceinline__ void nearest_site(const Shared<REF64, GW>& sh, const Dims& dm, int slot, int thin,
                                              int crown, int& nearest, float& xdist) {
   const float base = posf<REF64>(sh, crown);
   const float head = base + XB_REACH;
   const int cnt = sh.fcount[slot];
-  const LDS short* cs = sh.fsites + slot * dm.W;
+  const auto cs = sh.fsites + slot * dm.W;
   const int tb = dm.nk + thin * dm.ns;
   float best = INFINITY;
   int bj = 0;
@@ -940,7 +965,7 @@ __device__ __forceinline__ void wave_count(LDS int* c, bool pred, int lane) {
   if (lane == 0 && m) atomicAdd((int*)c, (int)__popcll(m));
 }
 
-template <int BS, int MS, int MX, bool REF64, bool DYN>
+template <int BS, int MS, int MX, bool REF64, bool DYN, bool GW>
 __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   extern __shared__ __align__(16) char smem[];
   constexpr int NW = BS / WAVE;
@@ -959,7 +984,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   dm.W = G.width; dm.nf = G.n_faces; dm.nb = G.n_bins;
   dm.nk = dm.Tk * dm.nc; dm.N = dm.nk + dm.Tn * dm.ns; dm.X = dm.nk * dm.nx; dm.S = dm.Tn * dm.ns;
   dm.nch = dm.Tk + dm.Tn;
-  Shared<REF64> sh = carve<BS, REF64>((LDS char*)smem, dm);
+  // GW variants: this instance's slice of the device workspace
+  char* ws = GW ? (char*)L.workspace + (int64_t)b * L.workspace_stride : nullptr;
+  Shared<REF64, GW> sh = carve<BS, REF64, GW>((LDS char*)smem, ws, dm);
   Red<BS> rd{sh.red, 0};
 
   const float* lf = L.lat_f32;
@@ -1465,14 +1492,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
 // ---------------------------------------------------------------------------
 struct Variant {
   int bs, ms, mx;
-  bool ref64, dyn;
+  bool ref64, dyn, gw;
   const void* fn;
   int64_t (*lds)(const Dims&);
+  int64_t (*ws)(const Dims&);
 };
 
-template <int BS, int MS, int MX, bool R, bool D = false>
+template <int BS, int MS, int MX, bool R, bool D = false, bool GW = false>
 Variant make_variant() {
-  return Variant{BS, MS, MX, R, D, (const void*)&run_kernel<BS, MS, MX, R, D>, &lds_bytes<BS, R>};
+  return Variant{BS, MS, MX, R, D, GW, (const void*)&run_kernel<BS, MS, MX, R, D, GW>, &lds_bytes<BS, R, GW>,
+                 &ws_bytes<BS, R, GW>};
 }
 
 // Fixed-spacing variants first; the dynamic-spacing (bordered solve) ones
@@ -1490,6 +1519,9 @@ const Variant* variants(int& n) {
       make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
       make_variant<256, 8, 4, false>(),  make_variant<256, 8, 4, true>(),
       make_variant<256, 16, 16, false>(), make_variant<256, 16, 16, true>(),
+      // global-workspace variants, for lattices too large for LDS (6x6)
+      make_variant<1024, 16, 8, false, false, true>(), make_variant<1024, 16, 8, true, false, true>(),
+      make_variant<1024, 16, 8, false, true, true>(),  make_variant<1024, 16, 8, true, true, true>(),
   };
   n = (int)(sizeof(v) / sizeof(v[0]));
   return v;
@@ -1516,7 +1548,11 @@ void needs(const Dims& d, int bs, int& ms, int& mx) {
   mx = (d.X + bs - 1) / bs;
 }
 
-int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant** out, int64_t* lds) {
+// First variant (LDS-resident ones come first) that fits every lattice of
+// the launch; *lds its dynamic LDS bytes, *ws its workspace bytes per
+// instance (0 unless it is a global-workspace variant).
+int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant** out, int64_t* lds,
+         int64_t* ws) {
   int n;
   const Variant* v = variants(n);
   const bool r64 = L->cfg.mode == MF_SOLVER_REFINED;
@@ -1524,7 +1560,7 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
   for (int i = 0; i < n; ++i) {
     if (v[i].ref64 != r64 || v[i].dyn != dyn || v[i].bs != L->block) continue;
     bool ok = true;
-    int64_t need = 0;
+    int64_t need = 0, wneed = 0;
     for (const auto& G : lats) {
       const Dims d = dims_of(G);
       int ms, mx;
@@ -1534,8 +1570,14 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
       // the bordered solve keeps d f / d d per head in the [N] CG buffer
       if (v[i].dyn && d.X > d.N) ok = false;
       need = std::max(need, v[i].lds(d));
+      wneed = std::max(wneed, v[i].ws(d));
     }
-    if (ok && need <= 160 * 1024) { *out = &v[i]; *lds = need; return 0; }
+    if (ok && need <= 160 * 1024) {
+      *out = &v[i];
+      *lds = need;
+      if (ws) *ws = wneed;
+      return 0;
+    }
   }
   return fail("no kernel variant fits this lattice at block " + std::to_string(L->block) +
               " (LDS or register slots exceeded)");
@@ -1587,7 +1629,16 @@ int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes) {
   std::vector<mf_lattice> lats;
   if (int e = fetch_lattices(L, lats)) return e;
   const Variant* v;
-  return pick(L, lats, &v, lds_bytes);
+  return pick(L, lats, &v, lds_bytes, nullptr);
+}
+
+int mf_workspace_bytes(const mf_launch* L, int64_t* per_instance) {
+  if (!L || !per_instance) return fail("null argument");
+  std::vector<mf_lattice> lats;
+  if (int e = fetch_lattices(L, lats)) return e;
+  const Variant* v;
+  int64_t lds;
+  return pick(L, lats, &v, &lds, per_instance);
 }
 
 int mf_run(const mf_launch* L, void* stream) {
@@ -1598,8 +1649,11 @@ int mf_run(const mf_launch* L, void* stream) {
   std::vector<mf_lattice> lats;
   if (int e = fetch_lattices(L, lats)) return e;
   const Variant* v;
-  int64_t lds;
-  if (int e = pick(L, lats, &v, &lds)) return e;
+  int64_t lds, ws = 0;
+  if (int e = pick(L, lats, &v, &lds, &ws)) return e;
+  if (ws > 0 && (L->workspace == nullptr || L->workspace_stride < ws))
+    return fail("this launch needs a device workspace of " + std::to_string(ws) +
+                " bytes per instance (mf_workspace_bytes)");
   HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   void* args[] = {(void*)L};
   HIPCHK(hipLaunchKernel(v->fn, dim3(L->B), dim3(v->bs), args, (size_t)lds, (hipStream_t)stream));

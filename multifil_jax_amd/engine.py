@@ -223,6 +223,7 @@ class Batch:
         self.d_force = torch.zeros((self.B, self.T), dtype=torch.float64, device=dev)
         self._inst_cache = {}
         self.d_inst = None
+        self.d_ws = None
         self.block = default_block(pack.topos)
         self.initialized = np.zeros(self.B, dtype=bool)
         self._table_rows = None      # (B, 2) pCa / spacing the tables were built for
@@ -276,6 +277,18 @@ class Batch:
         L.lattices_host = C.addressof(self.pack.desc_host)
         return L
 
+    def _workspace(self, L):
+        """Attach a device workspace when the chosen kernel keeps per-instance
+        arrays in device memory (lattices too large for LDS)."""
+        need = C.c_int64(0)
+        _lib.check(self.lib.mf_workspace_bytes(C.byref(L), C.byref(need)), "mf_workspace_bytes")
+        if need.value > 0:
+            total = need.value * self.B
+            if self.d_ws is None or self.d_ws.numel() < total:
+                self.d_ws = _torch().empty(total, dtype=_torch().uint8, device=self.device)
+            L.workspace = self.d_ws.data_ptr()
+            L.workspace_stride = need.value
+
     def _stream(self):
         torch = _torch()
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -326,6 +339,7 @@ class Batch:
                 _lib.check(self.lib.mf_tables(C.byref(L), stream), "mf_tables")
                 self._table_rows = rows.copy()
                 self._table_dt = cfg.dt
+            self._workspace(L)
             if kernel_events is not None and s0 == segs[0][0]:
                 kernel_events[0].record(_torch().cuda.current_stream(self.device))
             _lib.check(self.lib.mf_run(C.byref(L), stream), "mf_run")

@@ -1,0 +1,45 @@
+"""run() end to end on the GPU: the sweep it builds runs through the engine
+and reshapes into the reference's data cube (simulation.py:1098-1165)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    return torch
+
+
+def test_run_matches_direct_batch(torch_cuda, topo22, skeletal):
+    from multifil_jax_amd import run
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    from multifil_jax_amd.simulation import plan_sweep
+    s, d = skeletal
+    res = run(topo22, duration_ms=12, pCa=[4.5, 6.0], replicates=2, rng_seed=3, solver="refined",
+              static_params=s)
+    assert res.axial_force.shape == (2, 2, 12)
+    assert res._axis_names == ['pCa', 'replicates', 'time']
+    assert len(res.metrics) == 52
+    plan = plan_sweep(topo22, duration_ms=12, pCa=[4.5, 6.0], replicates=2)
+    seeds = (np.uint64(3) << np.uint64(32)) | np.arange(4, dtype=np.uint64)
+    bt = Batch(LatticePack([topo22]), np.zeros(4, np.int32), plan.params, plan.drivers, seeds=seeds)
+    bt.run(12, cfg=SolverConfig(mode="refined"))
+    assert np.array_equal(res.axial_force.reshape(4, 12), bt.force())
+    assert np.array_equal(res.metrics['n_bound'].reshape(4, 12), bt.metrics()[:, :, 5])
+    # activation: the pCa 4.5 rows bind, the pCa 6.0 rows hardly do
+    assert res.sel(pCa=4.5).metrics['n_bound'][:, -1].mean() > res.sel(pCa=6.0).metrics['n_bound'][:, -1].mean()
+    assert res.mean().axial_force.shape == (2, 12)
+
+
+def test_run_dynamic_spacing(torch_cuda, topo22, skeletal):
+    from multifil_jax_amd import run
+    s, _ = skeletal
+    res = run(topo22, duration_ms=8, pCa=4.5, z_line=1000.0, K_lat=5.0, solver="refined", static_params=s)
+    ls = res.metrics['lattice_spacing'][0]
+    assert res.topology_config['K_lat_eff'] == 5.0 * topo22.n_thick
+    assert np.all(np.isfinite(ls)) and np.ptp(ls) > 0.0
+    assert np.all(res.metrics['solver_residual'] <= 1e-5)

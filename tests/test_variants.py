@@ -20,9 +20,10 @@ def lib():
         pytest.skip(f"engine library not built: {e}")
 
 
-def _lds(lib, topos, mode, block):
+def _lds(lib, topos, mode, block, dynamic=False, ws=False):
     pack = LatticePack(topos)
     L = _lib.MfLaunch()
+    L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if dynamic else 0
     L.n_lattices = len(topos)
     L.B = 1
     L.block = block
@@ -30,18 +31,25 @@ def _lds(lib, topos, mode, block):
     L.cfg.n_cg = 6
     L.lattices_host = C.addressof(pack.desc_host)
     out = C.c_int64(0)
-    rc = lib.mf_run_lds_bytes(C.byref(L), C.byref(out))
+    fn = lib.mf_workspace_bytes if ws else lib.mf_run_lds_bytes
+    rc = fn(C.byref(L), C.byref(out))
     return rc, out.value
 
 
-@pytest.mark.parametrize("n", [2, 3, 4])
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("n", [2, 3, 4, 6])
 @pytest.mark.parametrize("mode", [_lib.MF_SOLVER_REFERENCE, _lib.MF_SOLVER_REFINED])
-def test_variant_fits(lib, skeletal, n, mode):
+def test_variant_fits(lib, skeletal, n, mode, dynamic):
     s, d = skeletal
     topo = SarcTopology.create(n, n, s, d)
-    rc, lds = _lds(lib, [topo], mode, default_block([topo]))
+    blk = default_block([topo])
+    rc, lds = _lds(lib, [topo], mode, blk, dynamic)
     assert rc == 0, lib.mf_last_error().decode()
     assert 0 < lds <= LDS_PER_CU
+    rc, ws = _lds(lib, [topo], mode, blk, dynamic, ws=True)
+    assert rc == 0
+    # only the 6x6 lattice needs its per-XB / per-site arrays in device memory
+    assert (ws > 0) == (n == 6)
 
 
 def test_mixed_lattices_share_a_variant(lib, skeletal):
