@@ -1,0 +1,100 @@
+"""Precision check on a 4x4 lattice (16 thick, 32 thin filaments): 500 steps at
+pCa 5.5, z_line 1000 nm (the initial state equilibrated against the
+parameter row's 900 nm Z-line, as the reference's run() does), fixed spacing
+14 nm, Philox seed 0, refined solver, against the FP64 oracle.
+
+Teacher-forced: the oracle's state after every step t (t = 0..499) is
+loaded into its own instance and advanced one step with step t's draws, all
+500 in one launch; each result is compared with the oracle's state t+1.
+Bars: identical discrete state; node positions within 1e-6 nm (1e-9
+relative to the ~1000 nm coordinates); M-line force within 1e-7 relative.
+Free run: 500 device steps with the same draws track the oracle's
+trajectory; up to the first step whose XB state counts differ (a sampling
+draw that lands within rounding of a table boundary) the force stays within
+1e-5 relative, and that prefix is at least 100 steps.
+"""
+import numpy as np
+import pytest
+
+from oracle import mf_oracle as mo
+from oracle.philox import uniforms, PURPOSE_TM, PURPOSE_XB
+
+from gpu_helpers import oracle_state_to_device, drivers_const
+
+pytestmark = pytest.mark.gpu
+
+T, PCA, Z, SEED = 500, 5.5, 1000.0, 0
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def setup(skeletal, pdict):
+    from multifil_jax_amd.topology import SarcTopology
+    s, d = skeletal
+    topo = SarcTopology.create(4, 4, s, d)
+    cfg = mo.SolverConfig(mode="exact")
+    pre = mo.preconditioner(topo, pdict)
+    st = mo.init_state(topo, pdict, Z, cfg, pre)
+    st.ls = 14.0
+    tab = mo.make_tables(pdict, PCA, 14.0, 1.0, topo.xb_bin_centers)
+    S, X = topo.n_thin * topo.n_sites, topo.total_xbs
+    states, forces = [st.copy()], []
+    for t in range(T):
+        st, mt = mo.step(st, topo, pdict, tab, Z, PCA, 14.0, 0.0, uniforms(SEED, t, PURPOSE_TM, S),
+                         uniforms(SEED, t, PURPOSE_XB, X), cfg, pre)
+        states.append(st.copy())
+        forces.append(mt['axial_force'])
+    return topo, d, states, np.array(forces)
+
+
+def test_teacher_forced_500_steps(torch_cuda, setup):
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    topo, d, states, forces = setup
+    S, X = topo.n_thin * topo.n_sites, topo.total_xbs
+    bt = Batch(LatticePack([topo]), np.zeros(T, np.int32), d.rows(T), drivers_const(T, 1, Z, PCA),
+               seeds=np.zeros(T, np.uint64))
+    U = np.zeros((T, 1, S + X), np.float32)
+    for t in range(T):
+        bt.set_state(t, oracle_state_to_device(states[t], topo))
+        U[t, 0, :S] = uniforms(SEED, t, PURPOSE_TM, S)
+        U[t, 0, S:] = uniforms(SEED, t, PURPOSE_XB, X)
+    bt.run(1, cfg=SolverConfig(mode="refined"), uniforms=torch_cuda.from_numpy(U).cuda())
+    fg = bt.force()[:, 0]
+    dx_max = rel_f = 0.0
+    n_same = 0
+    for t in range(T):
+        g, want = bt.get_state(t), states[t + 1]
+        if not (np.array_equal(g["xb_state"], want.xb_state) and np.array_equal(g["tm"], want.tm.ravel())):
+            continue
+        n_same += 1
+        dx_max = max(dx_max, float(np.max(np.abs(g["pos"] - want.x()))))
+        rel_f = max(rel_f, abs(fg[t] - forces[t]) / abs(forces[t]))
+    print(f"\n4x4 precision, teacher-forced: {n_same}/{T} steps with identical discrete state; "
+          f"max |dx| {dx_max:.3e} nm; max force rel err {rel_f:.3e}")
+    assert n_same >= T - 1
+    assert dx_max < 1e-6
+    assert rel_f < 1e-7
+
+
+def test_free_run_500_steps(torch_cuda, setup):
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    topo, d, states, forces = setup
+    bt = Batch(LatticePack([topo]), np.zeros(1, np.int32), d.rows(1), drivers_const(1, T, Z, PCA),
+               seeds=np.array([SEED], np.uint64))
+    bt.run(T, cfg=SolverConfig(mode="refined"))
+    fg, mg = bt.force()[0], bt.metrics()[0]
+    counts = np.array([np.bincount(s.xb_state, minlength=6) for s in states[1:]])
+    same = np.all(mg[:, 6:12] == counts, axis=1)
+    first_diff = int(np.argmin(same)) if not same.all() else T
+    rel = np.abs(fg - forces) / np.abs(forces)
+    print(f"\n4x4 free run: state counts agree for the first {first_diff} steps; "
+          f"max force rel err over them {rel[:first_diff].max():.3e}")
+    assert first_diff >= 100
+    assert rel[:first_diff].max() < 1e-5
