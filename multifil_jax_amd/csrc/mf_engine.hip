@@ -461,16 +461,22 @@ __device__ __forceinline__ Springs springs(const LDS float* prm, int st) {
   return s;
 }
 
-// Axial force of a bound XB at offset xd (forces.py:366-399) and its
-// derivative d f / d xd (what jax.jvp produces at solver.py:413).
-// In double the angle comes from a float atan2 (|error| ~1e-7 rad, i.e.
-// ~2e-7 pN per head, far below the refined solver's stopping tolerance); the
-// offsets, lengths and products stay in double.
+// Angle of a bound head, theta = atan2(d, xd).  The fixed-spacing refined
+// residual takes it from a float atan2 (|error| ~1e-7 rad, ~2e-7 pN per
+// head, far below the stopping tolerance at any one node).  The radial
+// residual of dynamic spacing sums over every bound head of the lattice,
+// where that rounding would add up to ~1e-5 pN, so it uses a double atan2.
+template <typename T, bool EXACT_ANGLE>
+__device__ __forceinline__ T xb_angle(T d, T xd) {
+  if constexpr (EXACT_ANGLE) return atan2(d, xd); else return T(atan2f((float)d, (float)xd));
+}
+
+// Axial force of a bound XB at offset xd and angle th (forces.py:366-399)
+// and its derivative d f / d xd (what jax.jvp produces at solver.py:413).
 template <typename T>
-__device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, float& kf) {
+__device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T th, T& f, float& kf) {
   const T r = sqrt(xd * xd + d * d);
   const T rs = r > T(1e-10) ? r : T(1e-10);
-  const T th = T(atan2f((float)d, (float)xd));
   const T cs = xd / rs, sn = d / rs;
   f = T(sp.gk) * (r - T(sp.gr)) * cs - (T(1) / rs) * T(sp.ck) * (th - T(sp.cr)) * sn;
   const T r2 = rs * rs;
@@ -483,10 +489,9 @@ __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T& f, flo
 // the radial force g (forces.py:708-709) and g_d = d g / d d.  With
 // E the head's energy, f = dE/dxd and g = dE/dd, so d g / d xd = f_d.
 template <typename T>
-__device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, float& f_d, T& g, float& g_d) {
+__device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, T th, float& f_d, T& g, float& g_d) {
   const T r = sqrt(xd * xd + d * d);
   const T rs = r > T(1e-10) ? r : T(1e-10);
-  const T th = T(atan2f((float)d, (float)xd));
   const T r3 = rs * rs * rs, r4 = r3 * rs;
   const T dth = th - T(sp.cr);
   f_d = (float)(T(sp.gk) * T(sp.gr) * xd * d / r3 - T(sp.ck) * (xd * d + dth * (xd * xd - d * d)) / r4);
@@ -600,11 +605,12 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       const int crown = j / dm.nx;
       const T xd = T(posd<REF64>(sh, site)) - T(posd<REF64>(sh, crown));
       const Springs sp = springs(prm, st);
-      xb_force<T>(sp, xd, d, fj, kj);
+      const T th = dyn ? xb_angle<T, true>(d, xd) : xb_angle<T, false>(d, xd);
+      xb_force<T>(sp, xd, d, th, fj, kj);
       if (dyn) {
         T g;
         float gd;
-        xb_radial<T>(sp, xd, d, fd, g, gd);
+        xb_radial<T>(sp, xd, d, th, fd, g, gd);
         sg += g;
         sgd += T(gd);
       }

@@ -35,7 +35,7 @@ def topo66(skeletal):
 def test_6x6_free_run_refined(torch_cuda, topo66, skeletal, pdict, dynamic):
     from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
     _, d = skeletal
-    T = 8
+    T = 24   # long enough to reach full activation, where the radial sum is largest
     drv = drivers_const(1, T, 1000.0, 4.5)
     K = 5.0 * topo66.n_thick
     bt = Batch(LatticePack([topo66]), np.zeros(1, np.int32), d.rows(1), drv, seeds=np.array([5], np.uint64),
