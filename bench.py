@@ -204,7 +204,7 @@ def uneven_conditions(seed=20261203, per_lattice=96, lattices=(2, 3, 4, 6)):
     return lat, lengths, pCa
 
 
-def workload_uneven(cfg, block=None):
+def workload_uneven(cfg, block=None, autotune=True):
     """96 instances each of 2x2 / 3x3 / 4x4 / 6x6, lengths U{200..1000},
     dynamic spacing.  The LDS-resident lattices share one launch; 6x6 (device
     workspace) runs in a second launch on its own stream, concurrently."""
@@ -235,13 +235,16 @@ def workload_uneven(cfg, block=None):
                    dynamic=(K, np.zeros(B, np.float32)), lengths=lengths[idx])
         if block:
             bt.block = block
+        elif autotune:
+            from multifil_jax_amd.autotune import tune
+            bt.block, bt.tuned_cfg = tune(bt, mode=cfg.mode)
         batches.append(bt)
     streams = [torch.cuda.Stream() for _ in batches]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for bt, st in zip(batches, streams):
         with torch.cuda.stream(st):
-            bt.run(T, cfg=cfg, sync=False)
+            bt.run(T, cfg=getattr(bt, "tuned_cfg", cfg), sync=False)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     total = int(lengths.sum())
@@ -254,7 +257,9 @@ def workload_uneven(cfg, block=None):
             "config": {"workload": "uneven: 96 each of 2x2/3x3/4x4/6x6, lengths U{200..1000}, pCa U[4.5,6.5], "
                                    "z_line 900 nm, dynamic spacing K_lat 5 pN/nm per thick, nu 0, d0 14 nm, "
                                    "dt 1 ms; default_rng(20261203)",
-                       "solver": cfg.mode, "launches": 2}}
+                       "solver": cfg.mode, "launches": 2,
+                       "groups": [{"block": bt.block, "n_cg": getattr(bt, "tuned_cfg", cfg).n_cg,
+                                   "instances": bt.B} for bt in batches]}}
 
 
 def workload_sinusoid(cfg):
@@ -301,6 +306,8 @@ def main():
     ap.add_argument("--n-cg", type=int, default=6, help="CG iterations per Newton step")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/oracle/cpu legs")
     ap.add_argument("--workload", default="sweep", choices=["sweep", "plain", "uneven", "sinusoid"])
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the block-size / CG-cap autotuner (sweep and uneven workloads)")
     ap.add_argument("--copy-probe", action="store_true",
                     help=f"after timing, copy {COPY_PROBE_MIB} MiB on the device (counter calibration)")
     args = ap.parse_args()
@@ -312,17 +319,24 @@ def main():
     from multifil_jax_amd.engine import SolverConfig
     cfg = SolverConfig(mode=args.mode, n_cg=args.n_cg)
     if args.workload != "sweep":
-        fn = {"plain": workload_plain, "uneven": lambda c: workload_uneven(c, args.block),
+        fn = {"plain": workload_plain,
+              "uneven": lambda c: workload_uneven(c, args.block, not args.no_autotune),
               "sinusoid": workload_sinusoid}[args.workload]
         print(json.dumps(fn(cfg)))
         return
     B, K, W = args.batch, args.steps, args.warmup
     bt, topo, d, drv = make_batch(B, W + K)
+    tuned = None
     if args.block:
         bt.block = args.block
+    elif not args.no_autotune:
+        from multifil_jax_amd.autotune import tune
+        bt.block, cfg = tune(bt, mode=args.mode)
+        tuned = {"block": bt.block, "n_cg": cfg.n_cg}
     # warmup: equilibration + W steps (also builds the transition tables)
     bt.run(W, t0=0, cfg=cfg)
     torch.cuda.synchronize()
+    bt.prepare(K, t0=W)                  # instance tables on the device before timing
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -350,7 +364,7 @@ def main():
         "config": {"workload": "batched sweep: 256 conditions on a 2x2 lattice, pCa U[4,7], "
                                "z_line U[850,1150] nm, dt 1 ms, seeds 0..255, fixed spacing 14 nm",
                    "lattice": "2x2", "batch": B, "solver": args.mode, "block": bt.block,
-                   "n_cg": args.n_cg},
+                   "n_cg": cfg.n_cg, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "run_kernel", "kernel_ms": ms_kernel, "launches": 1},

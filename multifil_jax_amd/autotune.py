@@ -1,0 +1,100 @@
+"""Autotuner for the step kernel's launch shape and CG cap.
+
+Two knobs decide the speed of a batch:
+
+* **Workgroup size.** It sets how many filament chains each wave owns, and
+  the register budget per lane (768 threads: 168 VGPRs; 1024: 128).
+* **CG iterations per Newton iteration.** Too few and Newton iterates more;
+  too many and each iteration costs more.
+
+The tuner runs short trial launches of the same instances on a copy of the
+batch. It tries every block size with a kernel variant for these lattices,
+times each (block, n_cg) pair with HIP events, and keeps the fastest one
+whose refined residual met its tolerance at every trial step. Choices are
+cached per (lattice shapes, solver mode, dynamic spacing) for the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .engine import Batch, SolverConfig, default_block
+
+BLOCKS = (512, 768, 1024)
+N_CG = (4, 6, 8, 10, 12)
+
+_cache: Dict[tuple, Tuple[int, int]] = {}
+
+
+def _key(batch: Batch, mode: str) -> tuple:
+    shapes = tuple(sorted({(t.n_thick, t.n_crowns, t.n_thin, t.n_sites) for t in batch.pack.topos}))
+    return shapes, mode, bool(batch.is_dynamic.any())
+
+
+def feasible_blocks(batch: Batch, mode: str) -> list:
+    """Block sizes for which a kernel variant fits every lattice of the batch."""
+    lib = _lib.load()
+    out = []
+    for blk in BLOCKS:
+        L = _lib.MfLaunch()
+        L.n_lattices = len(batch.pack.topos)
+        L.B = batch.B
+        L.block = blk
+        L.cfg = SolverConfig(mode=mode).ctype()
+        L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if batch.is_dynamic.any() else 0
+        L.lattices_host = C.addressof(batch.pack.desc_host)
+        lds = C.c_int64(0)
+        if lib.mf_run_lds_bytes(C.byref(L), C.byref(lds)) == 0:
+            out.append(blk)
+    return out
+
+
+def tune(batch: Batch, mode: str = "refined", trial_steps: int = 24, n_cg: Sequence[int] = N_CG,
+         blocks: Optional[Sequence[int]] = None, max_instances: int = 256, verbose: bool = False):
+    """Pick (block, n_cg) for ``batch``; returns (block, SolverConfig).
+
+    The trial copies up to ``max_instances`` instances (the longest-running
+    ones when lengths differ) and runs ``trial_steps`` steps after their
+    equilibration. The batch itself is left untouched."""
+    import torch
+    key = _key(batch, mode)
+    if key in _cache:
+        blk, ncg = _cache[key]
+        return blk, SolverConfig(mode=mode, n_cg=ncg)
+    T = min(trial_steps + 1, batch.T)
+    order = np.argsort(-batch.lengths, kind="stable")[:max_instances]
+    K = batch.K_lat[order] if batch.is_dynamic.any() else None
+    dyn = (K, batch.nu[order]) if K is not None else None
+    blocks = list(blocks) if blocks is not None else feasible_blocks(batch, mode)
+    best = None
+    for blk in blocks:
+        for ncg in n_cg:
+            cfg = SolverConfig(mode=mode, n_cg=ncg)
+            trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
+                          batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn)
+            trial.block = blk
+            trial.run(1, t0=0, cfg=cfg)                      # equilibrate + first step
+            s = torch.cuda.current_stream()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            trial.run(T - 1, t0=1, cfg=cfg, sync=False)
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            resid = float(trial.metrics()[:, 1:T, 1].max())
+            ok = mode != "refined" or resid <= cfg.refine_tol * (1 + 1e-6)
+            if verbose:
+                print(f"autotune block {blk} n_cg {ncg}: {ms / (T - 1) * 1e3:.1f} us/step, "
+                      f"max residual {resid:.2e}{'' if ok else ' (rejected)'}")
+            if ok and (best is None or ms < best[0]):
+                best = (ms, blk, ncg)
+            del trial
+    if best is None:
+        blk, ncg = default_block(batch.pack.topos), 6
+    else:
+        _, blk, ncg = best
+    _cache[key] = (blk, ncg)
+    return blk, SolverConfig(mode=mode, n_cg=ncg)

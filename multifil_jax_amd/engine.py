@@ -301,6 +301,14 @@ class Batch:
         cuts = [0] + [int(i) + 1 for i in np.nonzero(change)[0]] + [n]
         return [(t0 + a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
+    def prepare(self, n_steps: int, t0: int = 0):
+        """Build and upload the instance tables that run(n_steps, t0) will
+        use, so a timed run starts with every input already on the device."""
+        init = ~self.initialized
+        for (s0, n) in (self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]):
+            self._instances(n, init, s0)
+            init = np.zeros_like(init)
+
     def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
             block: Optional[int] = None, sync: bool = True, kernel_events=None):
         """Advance every instance from step t0 by n_steps (the first call

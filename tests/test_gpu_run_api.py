@@ -43,3 +43,21 @@ def test_run_dynamic_spacing(torch_cuda, topo22, skeletal):
     assert res.topology_config['K_lat_eff'] == 5.0 * topo22.n_thick
     assert np.all(np.isfinite(ls)) and np.ptp(ls) > 0.0
     assert np.all(res.metrics['solver_residual'] <= 1e-5)
+
+
+def test_autotune_picks_a_converging_config(torch_cuda, topo22, skeletal):
+    from multifil_jax_amd.autotune import tune, feasible_blocks
+    from multifil_jax_amd.engine import LatticePack, Batch
+    _, d = skeletal
+    B, T = 32, 40
+    drv = np.zeros((B, T, 3), np.float32)
+    drv[..., 0], drv[..., 1], drv[..., 2] = 1000.0, np.linspace(4.2, 6.5, B)[:, None], 14.0
+    bt = Batch(LatticePack([topo22]), np.zeros(B, np.int32), d.rows(B), drv, seeds=np.arange(B, dtype=np.uint64))
+    blocks = feasible_blocks(bt, "refined")
+    assert 768 in blocks and 1024 in blocks
+    blk, cfg = tune(bt, mode="refined", trial_steps=10)
+    assert blk in blocks and cfg.n_cg in (4, 6, 8, 10, 12)
+    assert not bt.initialized.any()           # the tuner ran on a copy
+    bt.block = blk
+    bt.run(T, cfg=cfg)
+    assert np.all(bt.metrics()[:, :, 1] <= 1e-5)
