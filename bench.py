@@ -123,28 +123,45 @@ def plain_gpu(cfg, n_cond=8, n_steps=100, block=None):
     return total / t_all, f"{n_cond} conditions x {n_steps} steps, one launch per step"
 
 
-def cpu_baseline(seconds=10.0):
-    """FP64 NumPy oracle, single core, bounded sample of the same workload."""
-    from oracle import mf_oracle as mo
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(B=256, T=1000, reps=3):
+    """The C restatement of the reference algorithm (oracle/cpu_port.c: FP32,
+    the reference solver's caps), OpenMP over conditions, on this host's
+    cores: the sweep's own condition table (B conditions x T steps), one
+    warm-up, best of ``reps``.  Transition tables are built before timing,
+    as on the GPU."""
+    from oracle import cpu_port
     from multifil_jax_amd.params import DYNAMIC_FIELDS
     from multifil_jax_amd.topology import SarcTopology
     from multifil_jax_amd.params import get_skeletal_params
     s, d = get_skeletal_params()
     topo = SarcTopology.create(2, 2, s, d)
     p = {n: float(getattr(d, n)) for n in DYNAMIC_FIELDS}
-    drv = conditions(256, 1)
-    steps = 0
-    t0 = time.perf_counter()
-    c = 0
-    while time.perf_counter() - t0 < seconds:
-        n = 50
-        mo.run(topo, p, np.full(n, drv[c, 0, 0]), np.full(n, drv[c, 0, 1]), np.full(n, 14.0),
-               1.0, c, mo.SolverConfig(mode="exact"))
-        steps += n
-        c += 1
-    el = time.perf_counter() - t0
-    return {"value": steps / el, "unit": "sarcomere-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{c} conditions x 50 steps (incl. equilibration), FP64 NumPy oracle, exact solve"}
+    threads = max(1, min(16, os.cpu_count() or 1))
+    drv = conditions(B, T)
+    seeds = np.arange(B, dtype=np.uint64)
+    prep = cpu_port.prepare(topo, p, drv)
+    cpu_port.run(topo, p, drv[:threads, :20], seeds[:threads], threads=threads,
+                 prepared={**prep, "tm": prep["tm"][:threads], "xb": prep["xb"][:threads]})
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cpu_port.run(topo, p, drv, seeds, threads=threads, prepared=prep)
+        el = time.perf_counter() - t0
+        best = el if best is None else min(best, el)
+    return {"value": B * T / best, "unit": "sarcomere-steps/s", "cores": threads, "kind": "port",
+            "sample": f"the full sweep ({B} conditions x {T} steps), C restatement (oracle/cpu_port.c, FP32, "
+                      f"reference solver caps), OpenMP {threads} threads on {cpu_model()}, best of {reps}"}
 
 
 def _timed(bt, n_steps, cfg, t0=0):
