@@ -224,6 +224,7 @@ class Batch:
         self._inst_cache = {}
         self.d_inst = None
         self.d_ws = None
+        self._changes = None
         self.block = default_block(pack.topos)
         self.initialized = np.zeros(self.B, dtype=bool)
         self._table_rows = None      # (B, 2) pCa / spacing the tables were built for
@@ -296,14 +297,18 @@ class Batch:
     def segments(self, t0: int, n: int) -> List[tuple]:
         """Split [t0, t0+n) where any instance's pCa or spacing changes: the
         transition tables are rebuilt between segments."""
-        pc = self.drivers_np[:, t0:t0 + n, 1:3]
-        change = np.any(np.any(pc[:, 1:] != pc[:, :-1], axis=2), axis=0)
-        cuts = [0] + [int(i) + 1 for i in np.nonzero(change)[0]] + [n]
-        return [(t0 + a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+        if self._changes is None:
+            # steps k whose pCa / spacing differ from step k-1 (found once)
+            pc = self.drivers_np[:, :, 1:3]
+            self._changes = np.nonzero(np.any(np.any(pc[:, 1:] != pc[:, :-1], axis=2), axis=0))[0] + 1
+        ch = self._changes[(self._changes > t0) & (self._changes < t0 + n)]
+        cuts = [t0] + [int(c) for c in ch] + [t0 + n]
+        return [(a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
     def prepare(self, n_steps: int, t0: int = 0):
         """Build and upload the instance tables that run(n_steps, t0) will
-        use, so a timed run starts with every input already on the device."""
+        use (and find the driver segments), so a timed run starts with every
+        input already on the device."""
         init = ~self.initialized
         for (s0, n) in (self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]):
             self._instances(n, init, s0)
