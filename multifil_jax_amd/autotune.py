@@ -4,8 +4,9 @@ Two knobs decide the speed of a batch:
 
 * **Workgroup size.** It sets how many filament chains each wave owns, and
   the register budget per lane (768 threads: 168 VGPRs; 1024: 128).
-* **CG iterations per Newton iteration.** Too few and Newton iterates more;
-  too many and each iteration costs more.
+* **CG iterations per Newton iteration** (refined mode only; reference mode
+  keeps the reference's 6). Too few and Newton iterates more; too many and
+  each iteration costs more.
 
 The tuner runs short trial launches of the same instances on a copy of the
 batch. It tries every block size with a kernel variant for these lattices,
@@ -52,19 +53,26 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
     return out
 
 
-def tune(batch: Batch, mode: str = "refined", trial_steps: int = 24, n_cg: Sequence[int] = N_CG,
-         blocks: Optional[Sequence[int]] = None, max_instances: int = 256, verbose: bool = False):
+def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_steps: int = 20,
+         n_cg: Sequence[int] = N_CG, blocks: Optional[Sequence[int]] = None, max_instances: int = 256,
+         verbose: bool = False):
     """Pick (block, n_cg) for ``batch``; returns (block, SolverConfig).
 
     The trial copies up to ``max_instances`` instances (the longest-running
-    ones when lengths differ) and runs ``trial_steps`` steps after their
-    equilibration. The batch itself is left untouched."""
+    ones when lengths differ), runs ``warmup_steps`` untimed steps past the
+    activation transient, then times ``trial_steps`` steps. The batch itself
+    is left untouched."""
     import torch
     key = _key(batch, mode)
     if key in _cache:
         blk, ncg = _cache[key]
         return blk, SolverConfig(mode=mode, n_cg=ncg)
-    T = min(trial_steps + 1, batch.T)
+    if mode != "refined":
+        # reference mode reproduces the reference's algorithm, whose CG count
+        # (6, params.py:656) is part of it: only the launch shape is tuned
+        n_cg = (6,)
+    W = min(warmup_steps, max(batch.T - 2, 1))
+    T = min(W + trial_steps, batch.T)
     order = np.argsort(-batch.lengths, kind="stable")[:max_instances]
     K = batch.K_lat[order] if batch.is_dynamic.any() else None
     dyn = (K, batch.nu[order]) if K is not None else None
@@ -76,18 +84,18 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 24, n_cg: Seque
             trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
                           batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn)
             trial.block = blk
-            trial.run(1, t0=0, cfg=cfg)                      # equilibrate + first step
+            trial.run(W, t0=0, cfg=cfg)                      # equilibrate + warm-up
             s = torch.cuda.current_stream()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(s)
-            trial.run(T - 1, t0=1, cfg=cfg, sync=False)
+            trial.run(T - W, t0=W, cfg=cfg, sync=False)
             e1.record(s)
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1)
-            resid = float(trial.metrics()[:, 1:T, 1].max())
+            resid = float(trial.metrics()[:, :T, 1].max())
             ok = mode != "refined" or resid <= cfg.refine_tol * (1 + 1e-6)
             if verbose:
-                print(f"autotune block {blk} n_cg {ncg}: {ms / (T - 1) * 1e3:.1f} us/step, "
+                print(f"autotune block {blk} n_cg {ncg}: {ms / (T - W) * 1e3:.1f} us/step, "
                       f"max residual {resid:.2e}{'' if ok else ' (rejected)'}")
             if ok and (best is None or ms < best[0]):
                 best = (ms, blk, ncg)

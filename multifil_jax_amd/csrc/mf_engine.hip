@@ -233,7 +233,7 @@ struct Shared {
   LDS float* tmcum;      // [27*16]
   LDS float* edges;      // [nb+1]
   LDS float* prm;        // [MF_N_PARAMS]
-  LDS float* red;        // [4][NW] scalar reduction ring
+  LDS float* red;        // [8][NW] scalar reduction ring
   LDS double* wpart;     // [NW][NWP] metric wave partials
   LDS double* fil;       // [Tk][NF] metric per-filament values
   LDS int* cnt;          // [NC] metric counters
@@ -279,7 +279,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.tmcum = take(4 * 27 * 16);
   L.edges = take(4LL * (d.nb + 1));
   L.prm = take(4 * MF_N_PARAMS);
-  L.red = take(4 * 4 * NW);
+  L.red = take(4 * 8 * NW);
   L.wpart = take(8LL * NW * NWP);
   L.fil = take(8LL * d.Tk * NF);
   L.cnt = take(4 * NC);
@@ -334,9 +334,9 @@ __device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, con
 template <int BS>
 struct Red {
   static constexpr int NW = BS / WAVE;
-  LDS float* buf;  // [4][NW]
+  LDS float* buf;  // [8][NW]
   int ring;
-  __device__ LDS float* row() { LDS float* p = buf + (ring & 3) * NW; ++ring; return p; }
+  __device__ LDS float* row() { LDS float* p = buf + (ring & 7) * NW; ++ring; return p; }
 };
 template <int BS>
 __device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave) {
@@ -701,38 +701,38 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
   return wg_max<BS>(rd, mx, lane, wave);
 }
 
-// -J p for the slot nodes (p read from LDS).
-template <int MS, bool REF64, bool GW>
+// -J p for the slot nodes, p read from the shared vector buffer pv.
+template <int MS, bool REF64, bool GW, typename V>
 __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                         const Partners<MS>& pt, const float (&ktip)[MS], float kk, float kn,
-                                        float (&Ap)[MS]) {
+                                        V pv, float (&Ap)[MS]) {
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     Ap[s] = 0.0f;
     const int n = sl.node[s];
     if (n < 0) continue;
     const int i = sl.li[s];
-    const float p0 = sh.pv[n];
+    const float p0 = pv[n];
     float q = 0.0f;  // sum over partners of k (p_partner - p0)
 #pragma unroll
     for (int h = 0; h < MAXP; ++h) {
       const int pk = pt.pk[s][h];
-      if (pk >= 0) q += pt.k[s][h] * (sh.pv[p_node(pk)] - p0);
+      if (pk >= 0) q += pt.k[s][h] * (pv[p_node(pk)] - p0);
     }
     float net;
     if (sl.thick[s]) {
-      const float pm = i > 0 ? sh.pv[n - 1] : 0.0f;
+      const float pm = i > 0 ? pv[n - 1] : 0.0f;
       const float lo = kk * (p0 - pm);
-      const float hi = i < dm.nc - 1 ? kk * (sh.pv[n + 1] - p0) : ktip[s] * p0;
+      const float hi = i < dm.nc - 1 ? kk * (pv[n + 1] - p0) : ktip[s] * p0;
       net = hi - lo + q;
     } else {
-      const float pp = i < dm.ns - 1 ? sh.pv[n + 1] : 0.0f;
+      const float pp = i < dm.ns - 1 ? pv[n + 1] : 0.0f;
       const float hi = kn * (pp - p0);
-      const float lo = i > 0 ? kn * (p0 - sh.pv[n - 1]) : 0.0f;
+      const float lo = i > 0 ? kn * (p0 - pv[n - 1]) : 0.0f;
       if (pt.overflow[s]) {
         int j = sh.shead[n - dm.nk];
         for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
-        for (; j >= 0; j = sh.xnext[j]) q += sh.kx[j] * (sh.pv[j / dm.nx] - p0);
+        for (; j >= 0; j = sh.xnext[j]) q += sh.kx[j] * (pv[j / dm.nx] - p0);
       }
       net = hi - lo + q;
     }
@@ -748,7 +748,11 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
 // arithmetic): u = M r and w = A u are formed first and (r.u, w.u) reduced
 // together, so each iteration costs one fused reduction and two barriers;
 // p and s = A p follow from u, w by the beta recurrence.  Refined mode only
-// needs the correction to converge: the FP64 residual decides when to stop.
+// needs the correction to converge: the FP64 residual decides when to stop,
+// and CG stops early once r.u reaches FP32 rounding.  (The pipelined variant
+// that shares one barrier between the reduction and the matvec was tried: its
+// recurred residual drifts from the true one once converged, and passive
+// instances, which converge in one or two passes, then lose accuracy.)
 //
 // With dynamic spacing the system is bordered: the unknown d (uniform, held
 // by every lane) couples to the nodes through the column cb, so
@@ -769,7 +773,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
 #pragma unroll
     for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; q[s] = 0.0f; }
     float r_d = dyn ? b_d : 0.0f, p_d = 0.0f, q_d = 0.0f;
-    float gamma = 0.0f, alpha = 1.0f;
+    float gamma = 0.0f, alpha = 1.0f, gamma0 = 0.0f;
     for (int it = 0;; ++it) {
       precond<MS>(r, u, sl, lane, ikk, ikn);
       const float u_d = rad.dinv * r_d;
@@ -777,7 +781,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       for (int s = 0; s < MS; ++s)
         if (sl.node[s] >= 0) sh.pv[sl.node[s]] = u[s];
       __syncthreads();
-      neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, w);
+      neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, sh.pv, w);
       float g = 0.0f, dl = 0.0f, cu = 0.0f;
       if (dyn) {
 #pragma unroll
@@ -794,6 +798,11 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       } else {
         wg_sum2<BS>(rd, g, dl, lane, wave);
       }
+      // converged to FP32 rounding (r.u ~ |r|^2 has fallen 14 orders) or
+      // broken down: further passes only add rounding noise (every lane sees
+      // the same reduced g, so the exit is uniform)
+      if (it == 0) gamma0 = g;
+      if (!(g > 1e-14f * gamma0)) break;
       const float beta = it == 0 ? 0.0f : g / (gamma + 1e-30f);
       const float den = it == 0 ? dl : dl - beta * g / alpha;
       alpha = g / (den + 1e-30f);
@@ -837,7 +846,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     }
     p_d = it == 0 ? z_d : z_d + beta * p_d;
     __syncthreads();
-    neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, Ap);
+    neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, sh.pv, Ap);
     loc = 0.0f;
     float pAp, Ap_d = 0.0f;
     if (dyn) {

@@ -55,7 +55,7 @@ def test_autotune_picks_a_converging_config(torch_cuda, topo22, skeletal):
     bt = Batch(LatticePack([topo22]), np.zeros(B, np.int32), d.rows(B), drv, seeds=np.arange(B, dtype=np.uint64))
     blocks = feasible_blocks(bt, "refined")
     assert 768 in blocks and 1024 in blocks
-    blk, cfg = tune(bt, mode="refined", trial_steps=10)
+    blk, cfg = tune(bt, mode="refined", trial_steps=10, warmup_steps=5)
     assert blk in blocks and cfg.n_cg in (4, 6, 8, 10, 12)
     assert not bt.initialized.any()           # the tuner ran on a copy
     bt.block = blk
