@@ -61,15 +61,19 @@ __device__ __forceinline__ float philox_uniform(uint64_t seed, int step, int pur
 // Inclusive prefix: Hillis-Steele inside each 16-lane row with row_shr:1/2/4/8,
 // then row_bcast:15 (rows 1, 3 take lane 15 of the row below) and
 // row_bcast:31 (rows 2, 3 take lane 31).  Totals are read from lane 63.
+// Each step is one DPP move plus the operation: lanes whose source lies
+// outside the row, and rows outside the row mask, keep the "old" operand,
+// which is the operation's identity (0 for sums, -inf for maxima), so no
+// lane predicate is needed.
 // ---------------------------------------------------------------------------
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_shift(float v, float id) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(v), CTRL, ROWS, 0xf, false));
 }
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_shift(double v, double id) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(id), __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(id), __double2hiint(v), CTRL, ROWS, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ float read_lane(float v, int l) {
@@ -80,27 +84,33 @@ __device__ __forceinline__ double read_lane(double v, int l) {
                           __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
-struct OpAdd { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
-struct OpMax { template <typename T> __device__ T operator()(T a, T b) const { return fmax(a, b); } };
+struct OpAdd {
+  template <typename T> __device__ T operator()(T a, T b) const { return a + b; }
+  template <typename T> __device__ static T identity() { return T(0); }
+};
+struct OpMax {
+  template <typename T> __device__ T operator()(T a, T b) const { return fmax(a, b); }
+  template <typename T> __device__ static T identity() { return -T(INFINITY); }
+};
 
 // inclusive prefix over the 64 lanes (lane 0 first)
 template <typename T, typename Op = OpAdd>
 __device__ __forceinline__ T wave_prefix(T v, int lane, Op op = Op()) {
-  const int rl = lane & 15;
-  T t;
-  t = dpp_mov<0x111>(v); if (rl >= 1) v = op(t, v);
-  t = dpp_mov<0x112>(v); if (rl >= 2) v = op(t, v);
-  t = dpp_mov<0x114>(v); if (rl >= 4) v = op(t, v);
-  t = dpp_mov<0x118>(v); if (rl >= 8) v = op(t, v);
-  t = dpp_mov<0x142>(v); if ((lane & 31) >= 16) v = op(t, v);
-  t = dpp_mov<0x143>(v); if (lane >= 32) v = op(t, v);
+  (void)lane;
+  const T id = Op::template identity<T>();
+  v = op(v, dpp_shift<0x111, 0xf>(v, id));   // row_shr:1
+  v = op(v, dpp_shift<0x112, 0xf>(v, id));   // row_shr:2
+  v = op(v, dpp_shift<0x114, 0xf>(v, id));   // row_shr:4
+  v = op(v, dpp_shift<0x118, 0xf>(v, id));   // row_shr:8
+  v = op(v, dpp_shift<0x142, 0xa>(v, id));   // row_bcast:15 into rows 1, 3
+  v = op(v, dpp_shift<0x143, 0xc>(v, id));   // row_bcast:31 into rows 2, 3
   return v;
 }
 // exclusive prefix (0 on lane 0), from the inclusive one shifted by a lane
 template <typename T>
 __device__ __forceinline__ T wave_prefix_excl(T incl, int lane) {
-  const T t = dpp_mov<0x138>(incl);  // wave_shr:1
-  return lane == 0 ? T(0) : t;
+  (void)lane;
+  return dpp_shift<0x138, 0xf>(incl, T(0));  // wave_shr:1; lane 0 has no source
 }
 // wave-uniform reductions
 template <typename T>

@@ -702,33 +702,40 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
 }
 
 // -J p for the slot nodes, p read from the shared vector buffer pv.
+// Branch-free per lane: a missing partner reads the node's own entry with
+// stiffness 0, neighbour indices are clamped into range and the chain ends
+// selected, so the wave runs one straight path (the filament type is
+// wave-uniform; only the rare over-full site list takes a branch).
 template <int MS, bool REF64, bool GW, typename V>
 __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                         const Partners<MS>& pt, const float (&ktip)[MS], float kk, float kn,
                                         V pv, float (&Ap)[MS]) {
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
-    Ap[s] = 0.0f;
-    const int n = sl.node[s];
-    if (n < 0) continue;
+    const int n0 = sl.node[s];
+    const int n = n0 >= 0 ? n0 : 0;
     const int i = sl.li[s];
     const float p0 = pv[n];
     float q = 0.0f;  // sum over partners of k (p_partner - p0)
 #pragma unroll
     for (int h = 0; h < MAXP; ++h) {
       const int pk = pt.pk[s][h];
-      if (pk >= 0) q += pt.k[s][h] * (pv[p_node(pk)] - p0);
+      const int pn = pk >= 0 ? p_node(pk) : n;
+      q += pt.k[s][h] * (pv[pn] - p0);
     }
+    const float pprev = pv[n > 0 ? n - 1 : 0];
     float net;
     if (sl.thick[s]) {
-      const float pm = i > 0 ? pv[n - 1] : 0.0f;
-      const float lo = kk * (p0 - pm);
-      const float hi = i < dm.nc - 1 ? kk * (pv[n + 1] - p0) : ktip[s] * p0;
+      const bool last = i >= dm.nc - 1;
+      const float pnext = pv[last ? n : n + 1];
+      const float lo = kk * (p0 - (i > 0 ? pprev : 0.0f));
+      const float hi = last ? ktip[s] * p0 : kk * (pnext - p0);
       net = hi - lo + q;
     } else {
-      const float pp = i < dm.ns - 1 ? pv[n + 1] : 0.0f;
-      const float hi = kn * (pp - p0);
-      const float lo = i > 0 ? kn * (p0 - pv[n - 1]) : 0.0f;
+      const bool last = i >= dm.ns - 1;
+      const float pnext = pv[last ? n : n + 1];
+      const float hi = kn * ((last ? 0.0f : pnext) - p0);
+      const float lo = i > 0 ? kn * (p0 - pprev) : 0.0f;
       if (pt.overflow[s]) {
         int j = sh.shead[n - dm.nk];
         for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
@@ -736,7 +743,7 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
       }
       net = hi - lo + q;
     }
-    Ap[s] = -net;
+    Ap[s] = n0 >= 0 ? -net : 0.0f;
   }
 }
 
