@@ -176,12 +176,18 @@ struct Slots {
 // XB partners of a lane's slot nodes, fixed during a solve: for a crown the
 // sites its bound heads hold, for a site the crowns of the heads bound to it
 // (list order).  Packed (partner DOF << 16 | XB index); -1 = none.  A site
-// with more than MAXP bound heads walks the LDS list instead (overflow).
+// with more than PM bound heads walks the LDS list instead (overflow).
+// Lanes that own at most two nodes keep PM = 4 partners per node in
+// registers; lanes with more nodes (large lattices) keep none (PM = 0) and
+// read a crown's heads and a site's list from LDS, since 4 partners x
+// their node count would not fit the register file.
 constexpr int MAXP = 4;
 template <int MS>
 struct Partners {
-  int pk[MS][MAXP];
-  float k[MS][MAXP];
+  static constexpr int PM = MS <= 2 ? MAXP : 0;
+  static constexpr int P = PM > 0 ? PM : 1;
+  int pk[MS][P];
+  float k[MS][P];
   bool overflow[MS];
 };
 __device__ __forceinline__ int p_node(int pk) { return pk >> 16; }
@@ -539,16 +545,21 @@ __device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& 
 template <int MS, bool REF64, bool GW>
 __device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                                Partners<MS>& pt) {
+  constexpr int PM = Partners<MS>::PM;
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     pt.overflow[s] = false;
 #pragma unroll
-    for (int h = 0; h < MAXP; ++h) { pt.pk[s][h] = -1; pt.k[s][h] = 0.0f; }
+    for (int h = 0; h < Partners<MS>::P; ++h) { pt.pk[s][h] = -1; pt.k[s][h] = 0.0f; }
     const int n = sl.node[s];
     if (n < 0) continue;
+    if (PM == 0) {
+      pt.overflow[s] = !sl.thick[s];  // sites walk their whole list
+      continue;
+    }
     if (sl.thick[s]) {
 #pragma unroll
-      for (int h = 0; h < MAXP; ++h) {
+      for (int h = 0; h < PM; ++h) {
         if (h >= dm.nx) break;
         const int j = n * dm.nx + h;
         const int site = sh.xsite[j];
@@ -558,7 +569,7 @@ __device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, cons
     } else {
       int j = sh.shead[n - dm.nk];
 #pragma unroll
-      for (int h = 0; h < MAXP; ++h) {
+      for (int h = 0; h < PM; ++h) {
         if (j < 0) break;
         pt.pk[s][h] = ((j / dm.nx) << 16) | j;
         j = sh.xnext[j];
@@ -633,14 +644,23 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
     const T x0 = T(posd<REF64>(sh, n));
     T xs = T(0);
     float cs = 0.0f;
+    constexpr int PM = Partners<MS>::PM;
 #pragma unroll
-    for (int h = 0; h < MAXP; ++h) {
+    for (int h = 0; h < PM; ++h) {
       const int pk = pt.pk[s][h];
       if (pk < 0) continue;
       const int j = p_xb(pk);
       if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
       pt.k[s][h] = sh.kx[j];
       if (dyn) cs += fdx[j];
+    }
+    if (PM == 0 && sl.thick[s]) {
+      // the crown's own heads (force and d f / d d are 0 for unbound ones)
+      for (int h = 0; h < dm.nx; ++h) {
+        const int j = n * dm.nx + h;
+        if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
+        if (dyn) cs += fdx[j];
+      }
     }
     T net;
     if (sl.thick[s]) {
@@ -670,7 +690,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       net = s_hi - s_lo;
       if (pt.overflow[s]) {
         int j = sh.shead[n - dm.nk];
-        for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
+        for (int h = 0; h < PM; ++h) j = sh.xnext[j];
         for (; j >= 0; j = sh.xnext[j]) {
           if constexpr (REF64) xs += T(sh.fx64[j]); else xs += T(sh.fx[j]);
           if (dyn) cs += fdx[j];
@@ -717,11 +737,20 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
     const int i = sl.li[s];
     const float p0 = pv[n];
     float q = 0.0f;  // sum over partners of k (p_partner - p0)
+    constexpr int PM = Partners<MS>::PM;
 #pragma unroll
-    for (int h = 0; h < MAXP; ++h) {
+    for (int h = 0; h < PM; ++h) {
       const int pk = pt.pk[s][h];
       const int pn = pk >= 0 ? p_node(pk) : n;
       q += pt.k[s][h] * (pv[pn] - p0);
+    }
+    if (PM == 0 && sl.thick[s]) {
+      // the crown's own heads (stiffness 0 for unbound ones)
+      for (int h = 0; h < dm.nx; ++h) {
+        const int j = n * dm.nx + h;
+        const int site = sh.xsite[j];
+        q += sh.kx[j] * (pv[site >= 0 ? site : n] - p0);
+      }
     }
     const float pprev = pv[n > 0 ? n - 1 : 0];
     float net;
@@ -738,7 +767,7 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
       const float lo = i > 0 ? kn * (p0 - pprev) : 0.0f;
       if (pt.overflow[s]) {
         int j = sh.shead[n - dm.nk];
-        for (int h = 0; h < MAXP; ++h) j = sh.xnext[j];
+        for (int h = 0; h < PM; ++h) j = sh.xnext[j];
         for (; j >= 0; j = sh.xnext[j]) q += sh.kx[j] * (pv[j / dm.nx] - p0);
       }
       net = hi - lo + q;
@@ -1535,7 +1564,9 @@ const Variant* variants(int& n) {
       make_variant<768, 2, 1, false>(),  make_variant<768, 2, 1, true>(),
       make_variant<768, 2, 1, false, true>(),  make_variant<768, 2, 1, true, true>(),
       make_variant<1024, 2, 1, false, true>(), make_variant<1024, 2, 1, true, true>(),
+      make_variant<1024, 5, 3, false, true>(), make_variant<1024, 5, 3, true, true>(),
       make_variant<1024, 8, 4, false, true>(), make_variant<1024, 8, 4, true, true>(),
+      make_variant<1024, 5, 3, false>(), make_variant<1024, 5, 3, true>(),
       make_variant<1024, 8, 4, false>(), make_variant<1024, 8, 4, true>(),
       make_variant<512, 4, 2, false>(),  make_variant<512, 4, 2, true>(),
       make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
