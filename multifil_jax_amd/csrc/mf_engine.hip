@@ -20,6 +20,11 @@
 
 using namespace mf;
 
+// Library-internal launch flag (above the public MF_LAUNCH_* bits): the
+// instance's XB sampling and absorbing-chain rows are copied into LDS after
+// its other arrays.
+constexpr int LAUNCH_TABLES_IN_LDS = 1 << 16;
+
 // Pointers into the workgroup's LDS carry the local address space so every
 // access compiles to ds_* instructions (a generic pointer would go through
 // flat addressing).
@@ -83,7 +88,7 @@ __device__ __forceinline__ void tm_table_entry(const float* prm, double pCa, dou
 // (rows 0 and 4 zeroed, transitions.py:1029-1078), the three entries the
 // metrics use.
 __device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, double d, double dt, double center,
-                                      int key, int nb, bool absorbing, float* tab) {
+                                      int key, int nb, bool absorbing, float* xbrows, float* absrows) {
   double Q[6][6];
   xb_generator<double>(prm, pCa, d, center, (double)(key / nb), absorbing, Q);
 #pragma unroll
@@ -92,7 +97,7 @@ __device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, dou
     for (int j = 0; j < 6; ++j) Q[i][j] *= dt;
   expm_ref<6, double>(Q);
   if (!absorbing) {
-    float* row = tab + TAB_XB + (int64_t)key * 48;
+    float* row = xbrows + (int64_t)key * 48;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       double c = 0.0;
@@ -102,7 +107,7 @@ __device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, dou
       row[i * 8 + 7] = 0.0f;
     }
   } else {
-    float* row = tab + tab_abs(nb) + (int64_t)key * 4;
+    float* row = absrows + (int64_t)key * 4;
     row[0] = (float)Q[3][4];
     row[1] = (float)Q[2][0];
     row[2] = (float)Q[3][0];
@@ -115,10 +120,11 @@ __device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, dou
 // registers than the step kernel around it may hold.
 __device__ __attribute__((noinline)) void xb_table_rebuild(const float* prm, float pCa, double d, float dt,
                                                            const float* centers, int nb, bool absorbing,
-                                                           float* tab, int tid, int nthreads) {
+                                                           float* xbrows, float* absrows, int tid,
+                                                           int nthreads) {
   for (int key = tid; key < 2 * nb; key += nthreads)
     xb_table_entry(prm, (double)pCa, (double)d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
-                   tab);
+                   xbrows, absrows);
 }
 
 __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_max) {
@@ -151,7 +157,8 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   if (absorbing) key -= 2 * nb_max;
   if (key >= 2 * nb) return;
   const float* centers = L.lat_f32 + G.bin_centers;
-  xb_table_entry(prm, (double)pCa, (double)d, dt, (double)centers[key % nb], key, nb, absorbing, tab);
+  xb_table_entry(prm, (double)pCa, (double)d, dt, (double)centers[key % nb], key, nb, absorbing, tab + TAB_XB,
+                 tab + tab_abs(nb));
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,8 +1172,20 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   int tmc = 0;  // which TM buffer is current
   const float inv_w = (float)nb / (lf[G.bin_edges + nb] - lf[G.bin_edges]);
   const double inv_nk = 1.0 / (double)dm.nk, inv_S = 1.0 / (double)S;
-  const float* xbtab = tab + TAB_XB;
-  const float* abstab = tab + tab_abs(nb);
+  // XB sampling and absorbing rows: in LDS when the launch's LDS budget
+  // holds them (the host sets LAUNCH_TABLES_IN_LDS and sizes the dynamic
+  // LDS), else read from the instance's table in device memory.  Read
+  // through generic pointers, which address either.
+  float* xbtab = tab + TAB_XB;
+  float* abstab = tab + tab_abs(nb);
+  if (L.launch_flags & LAUNCH_TABLES_IN_LDS) {
+    LDS float* tl = (LDS float*)(smem + lds_offsets<BS, REF64, GW>(dm).total);
+    const int nt = 2 * nb * 48 + 2 * nb * 4;
+    for (int i = tid; i < nt; i += BS) tl[i] = tab[TAB_XB + i];
+    xbtab = (float*)tl;
+    abstab = (float*)(tl + 2 * nb * 48);
+    __syncthreads();
+  }
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
 
   // ---- time loop; step -1 is create_and_equilibrate (simulation.py:633-642)
@@ -1197,7 +1216,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       if (DYN && rad.on) {
         // kinetics at the carried spacing (simulation.py:691-693): rebuild
         // the XB sampling rows; d_ref = ls (z0 / z)^nu (simulation.py:694-696)
-        xb_table_rebuild(prm_g, pCa, ls_carry, L.cfg.dt, centers, nb, false, tab, tid, BS);
+        xb_table_rebuild(prm_g, pCa, ls_carry, L.cfg.dt, centers, nb, false, xbtab, abstab, tid, BS);
         __syncthreads();
         rad.dref = lsd * powf(z_trace0 / zl, I.nu);
       }
@@ -1354,7 +1373,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     if (DYN && rad.on) {
       // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
       // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
-      xb_table_rebuild(prm_g, pCa, ls_new, L.cfg.dt, centers, nb, true, tab, tid, BS);
+      xb_table_rebuild(prm_g, pCa, ls_new, L.cfg.dt, centers, nb, true, xbtab, abstab, tid, BS);
       __syncthreads();
       float m_atpe = 0.0f, m_tear = 0.0f;
 #pragma unroll
@@ -1707,8 +1726,17 @@ int mf_run(const mf_launch* L, void* stream) {
   if (ws > 0 && (L->workspace == nullptr || L->workspace_stride < ws))
     return fail("this launch needs a device workspace of " + std::to_string(ws) +
                 " bytes per instance (mf_workspace_bytes)");
+  // transition rows into LDS when they fit beside the instance arrays
+  int64_t tab_bytes = 0;
+  for (const auto& G : lats) tab_bytes = std::max<int64_t>(tab_bytes, 4LL * (2LL * G.n_bins * 48 + 2LL * G.n_bins * 4));
+  mf_launch Lk = *L;
+  Lk.launch_flags &= ~LAUNCH_TABLES_IN_LDS;
+  if (lds + tab_bytes <= 160 * 1024) {
+    Lk.launch_flags |= LAUNCH_TABLES_IN_LDS;
+    lds += tab_bytes;
+  }
   HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  void* args[] = {(void*)L};
+  void* args[] = {(void*)&Lk};
   HIPCHK(hipLaunchKernel(v->fn, dim3(L->B), dim3(v->bs), args, (size_t)lds, (hipStream_t)stream));
   HIPCHK(hipGetLastError());
   return 0;
