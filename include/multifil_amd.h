@@ -96,7 +96,8 @@ typedef struct mf_solver_cfg {
   int32_t mode;        /* MF_SOLVER_*                                      */
   int32_t n_newton;    /* Newton cap (params.py:655: 4)                    */
   int32_t n_cg;        /* CG iterations per Newton step (params.py:656: 6) */
-  int32_t n_refine;    /* Newton cap in MF_SOLVER_REFINED                  */
+  int32_t n_refine;    /* Newton cap in MF_SOLVER_REFINED; up to 4x while  *
+                        * each iteration still halves max|F|             */
   float tol;           /* NaN: max(solver_tol, thick_k*1e-4, 0.25)         */
   float refine_tol;    /* pN, MF_SOLVER_REFINED stopping residual          */
   float dt;            /* ms                                               */

@@ -920,7 +920,11 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
 // returns the solved spacing, clamped at >= 1 nm after every Newton step
 // (solver.py:599-656).
 // Reference mode: FP32 residual, stop at max|F| <= tol or n_newton.
-// Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or n_refine.
+// Refined mode: FP64 residual, FP32 CG correction, stop at refine_tol or
+// n_refine iterations; past n_refine it continues (up to 4 n_refine) while
+// each iteration still halves max|F|, so a large Z-line jump (the first step
+// after the initial equilibration at the parameter row's Z-line) converges
+// and a residual stuck at a clamp does not run on.
 template <int BS, int MS, bool REF64, bool DYN, bool GW>
 __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                       const float (&rA)[MS], const float (&rB)[MS], float zl,
@@ -943,13 +947,15 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
   Partners<MS> pt;
   build_partners<MS, REF64>(sh, dm, sl, pt);
   int it = 0;
-  float mx;
+  float mx, prev = INFINITY;
   for (;;) {
     mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, f, ktip, cb, pt, rad, rd, tid, lane,
                                          wave);
     // the d-block preconditioner uses J_dd at the initial state (solver.py:633-638)
     if (DYN && rad.on && it == 0) rad.dinv = -1.0f / rad.jdd;
-    if (!(mx > tol && it < cap)) break;
+    if (!(mx > tol)) break;
+    if (it >= cap && !(REF64 && it < 4 * cap && mx < 0.5f * prev)) break;
+    prev = mx;
 #pragma unroll
     for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
     float dd;
@@ -1639,6 +1645,8 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
       needs(d, v[i].bs, ms, mx);
       int nchw = (d.nch + v[i].bs / WAVE - 1) / (v[i].bs / WAVE);
       if (ms > v[i].ms || mx > v[i].mx || nchw > 8 || d.N > 32767) ok = false;
+      // register partner lists hold at most MAXP heads per crown
+      if (v[i].ms <= 2 && d.nx > MAXP) ok = false;
       // the bordered solve keeps d f / d d per head in the [N] CG buffer
       if (v[i].dyn && d.X > d.N) ok = false;
       need = std::max(need, v[i].lds(d));

@@ -53,7 +53,8 @@ class SolverConfig:
     mode "reference": FP32 Newton-CG with the reference's caps (n_newton=4,
     n_cg=6; params.py:655-656) and tolerance floor (solver.py:714-718).
     mode "refined": the Newton residual in FP64 and the CG correction in FP32,
-    iterated until max|F| <= refine_tol pN (or n_refine iterations).
+    iterated until max|F| <= refine_tol pN, or n_refine iterations (up to
+    4 n_refine while each iteration still halves max|F|).
     """
     mode: str = "reference"
     n_newton: int = 4

@@ -52,6 +52,22 @@ def test_variant_fits(lib, skeletal, n, mode, dynamic):
     assert (ws > 0) == (n == 6)
 
 
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("mode", [_lib.MF_SOLVER_REFERENCE, _lib.MF_SOLVER_REFINED])
+@pytest.mark.parametrize("preset", ["lethocerus", "drosophila"])
+def test_invertebrate_variant_fits(lib, preset, mode, dynamic):
+    # 4 heads per crown, 80 candidates per head, 160 sites per strand
+    from multifil_jax_amd.params import get_lethocerus_params, get_drosophila_params
+    fn, shape = {"lethocerus": (get_lethocerus_params, (2, 2)),
+                 "drosophila": (get_drosophila_params, (2, 3))}[preset]
+    s, d = fn()
+    topo = SarcTopology.create(*shape, s, d)
+    assert topo.n_xb_per_crown == 4
+    rc, lds = _lds(lib, [topo], mode, default_block([topo]), dynamic)
+    assert rc == 0, lib.mf_last_error().decode()
+    assert 0 < lds <= LDS_PER_CU
+
+
 def test_mixed_lattices_share_a_variant(lib, skeletal):
     s, d = skeletal
     topos = [SarcTopology.create(n, n, s, d) for n in (2, 3, 4)]
