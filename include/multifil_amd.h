@@ -34,10 +34,11 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 3
+#define MF_ABI_VERSION 4
 #define MF_N_PARAMS 44      /* DynamicParams fields, params.py:73-553 order */
 #define MF_N_METRICS 52     /* metrics_fn.py:343-417 order                */
 #define MF_MAX_BINS 256     /* n_xb_bins upper bound                       */
+#define MF_MAX_POPS 4       /* subpopulations per instance                 */
 
 /* Solver modes. */
 #define MF_SOLVER_REFERENCE 0  /* FP32 Newton-CG, caps and tolerance floor of
@@ -48,6 +49,15 @@ extern "C" {
 /* Instance flags. */
 #define MF_INST_INIT    1   /* run create_and_equilibrate before step 0     */
 #define MF_INST_DYNAMIC 2   /* dynamic lattice spacing (K_lat, nu)           */
+
+/* Subpopulation modes (core/subpopulation.py; transitions.py:580-602,
+ * 925-975): populations share the mechanics and differ in their rates. */
+#define MF_POP_NONE       0
+#define MF_POP_MEAN_FIELD 1  /* one generator, sum_k f_k Q_k               */
+#define MF_POP_EXPLICIT   2  /* per-head / per-site labels pick table sets  */
+/* pop_flags: which kinetics the populations change (xb_* / tm_* scales). */
+#define MF_POP_XB 1
+#define MF_POP_TM 2
 
 /* Launch flags. */
 #define MF_LAUNCH_DYNAMIC 1 /* some instance has MF_INST_DYNAMIC: run the
@@ -89,7 +99,8 @@ typedef struct mf_instance {
   int64_t table_off;   /* float offset of the instance tables in `tables`  */
   float K_lat;         /* dynamic spacing: lattice stiffness x n_thick     */
   float nu;            /* dynamic spacing: Poisson exponent                */
-  int32_t pad0, pad1;
+  int64_t label_off;   /* MF_POP_EXPLICIT: byte offset of this instance's
+                          labels in pop_labels (u8 [X] XB then u8 [S] TM) */
 } mf_instance;
 
 typedef struct mf_solver_cfg {
@@ -129,6 +140,16 @@ typedef struct mf_launch {
                                    lattices whose per-instance arrays exceed
                                    LDS; NULL when mf_workspace_bytes gives 0 */
   int64_t workspace_stride;     /* bytes per instance, >= mf_workspace_bytes */
+  /* Subpopulations (all instances of a launch share the mode, K and flags,
+   * as run() requires, simulation.py:533-540). */
+  int32_t n_pops;               /* K (1 = none; <= MF_MAX_POPS)             */
+  int32_t pop_mode;             /* MF_POP_*                                 */
+  int32_t pop_flags;            /* MF_POP_XB | MF_POP_TM                    */
+  int32_t pad2;
+  const float* pop_params;      /* [B][K][MF_N_PARAMS] population constants,
+                                   row 0 = the wild type (= params)        */
+  const float* pop_fractions;   /* [B][K] mean-field weights               */
+  const uint8_t* pop_labels;    /* explicit: labels, see label_off          */
 } mf_launch;
 
 /* Byte layout of one instance state (all sections 8-byte aligned):
@@ -143,6 +164,10 @@ int64_t mf_state_bytes(int32_t n_thick, int32_t n_crowns, int32_t n_thin,
                        int32_t n_sites, int32_t n_xb_per_crown);
 /* Floats of per-instance table storage for n_bins. */
 int64_t mf_table_floats(int32_t n_bins);
+/* ... with n_pop_tm TM and n_pop_xb XB table sets (MF_POP_EXPLICIT: K for
+ * the kinetics the populations change, else 1).  Layout: header, TM sets,
+ * XB sets, absorbing sets; labels select the set. */
+int64_t mf_table_floats_pop(int32_t n_bins, int32_t n_pop_tm, int32_t n_pop_xb);
 
 int mf_abi_version(void);
 const char* mf_last_error(void);

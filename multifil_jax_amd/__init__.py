@@ -4,7 +4,7 @@ half-sarcomere simulator.
 Public names mirror the reference package (multifil_jax/__init__.py:80-100)
 for the parts this build provides: ``run``, ``SimulationResult``,
 ``SarcTopology``, ``get_skeletal_params`` / ``get_cardiac_params`` (plus the
-invertebrate presets), ``StaticParams`` / ``DynamicParams``.  The step itself
+invertebrate presets), ``StaticParams`` / ``DynamicParams``, ``Subpopulation``.  The step itself
 runs in hand-written gfx950 HIP kernels (``csrc/``) behind a C ABI
 (``include/multifil_amd.h``).
 """
@@ -12,6 +12,7 @@ from .params import (StaticParams, DynamicParams, Constants, get_skeletal_params
                      get_cardiac_params, get_lethocerus_params, get_drosophila_params,
                      DYNAMIC_FIELDS)
 from .topology import SarcTopology
+from .subpopulation import Subpopulation
 
 
 def __getattr__(name):
@@ -24,4 +25,4 @@ def __getattr__(name):
 
 __all__ = ["run", "SimulationResult", "SarcTopology", "StaticParams", "DynamicParams",
            "Constants", "get_skeletal_params", "get_cardiac_params", "get_lethocerus_params",
-           "get_drosophila_params", "DYNAMIC_FIELDS", "METRIC_NAMES"]
+           "get_drosophila_params", "DYNAMIC_FIELDS", "METRIC_NAMES", "Subpopulation"]

@@ -12,18 +12,24 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmultifil_amd.so")
 
-MF_ABI_VERSION = 3
+MF_ABI_VERSION = 4
 MF_N_PARAMS = 44
 MF_N_METRICS = 52
 MF_MAX_BINS = 256
+MF_MAX_POPS = 4
 MF_SOLVER_REFERENCE = 0
 MF_SOLVER_REFINED = 1
 MF_INST_INIT = 1
 MF_INST_DYNAMIC = 2
 MF_LAUNCH_DYNAMIC = 1
+MF_POP_NONE = 0
+MF_POP_MEAN_FIELD = 1
+MF_POP_EXPLICIT = 2
+MF_POP_XB = 1
+MF_POP_TM = 2
 
 EXPORTED = ("mf_abi_version", "mf_last_error", "mf_state_bytes", "mf_table_floats",
-            "mf_tables", "mf_run", "mf_run_lds_bytes", "mf_workspace_bytes")
+            "mf_table_floats_pop", "mf_tables", "mf_run", "mf_run_lds_bytes", "mf_workspace_bytes")
 
 
 class MfLattice(C.Structure):
@@ -40,7 +46,7 @@ class MfInstance(C.Structure):
     _fields_ = [("lattice", C.c_int32), ("n_steps", C.c_int32), ("t0", C.c_int32),
                 ("flags", C.c_int32), ("seed", C.c_uint64), ("state_off", C.c_int64),
                 ("table_off", C.c_int64), ("K_lat", C.c_float), ("nu", C.c_float),
-                ("pad0", C.c_int32), ("pad1", C.c_int32)]
+                ("label_off", C.c_int64)]
 
 
 class MfSolverCfg(C.Structure):
@@ -58,7 +64,10 @@ class MfLaunch(C.Structure):
                 ("u_stride", C.c_int32), ("cfg", MfSolverCfg), ("block", C.c_int32),
                 ("t_base", C.c_int32), ("launch_flags", C.c_int32), ("pad", C.c_int32),
                 ("lattices_host", C.c_void_p), ("workspace", C.c_void_p),
-                ("workspace_stride", C.c_int64)]
+                ("workspace_stride", C.c_int64),
+                ("n_pops", C.c_int32), ("pop_mode", C.c_int32), ("pop_flags", C.c_int32),
+                ("pad2", C.c_int32), ("pop_params", C.c_void_p), ("pop_fractions", C.c_void_p),
+                ("pop_labels", C.c_void_p)]
 
 
 _lib = None
@@ -84,6 +93,8 @@ def load() -> C.CDLL:
     lib.mf_state_bytes.argtypes = [C.c_int32] * 5
     lib.mf_table_floats.restype = C.c_int64
     lib.mf_table_floats.argtypes = [C.c_int32]
+    lib.mf_table_floats_pop.restype = C.c_int64
+    lib.mf_table_floats_pop.argtypes = [C.c_int32] * 3
     for fn in ("mf_tables", "mf_run"):
         getattr(lib, fn).restype = C.c_int
         getattr(lib, fn).argtypes = [C.POINTER(MfLaunch), C.c_void_p]

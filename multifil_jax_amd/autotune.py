@@ -12,7 +12,8 @@ The tuner runs short trial launches of the same instances on a copy of the
 batch. It tries every block size with a kernel variant for these lattices,
 times each (block, n_cg) pair with HIP events, and keeps the fastest one
 whose refined residual met its tolerance at every trial step. Choices are
-cached per (lattice shapes, solver mode, dynamic spacing) for the process.
+cached per (lattice shapes, solver mode, dynamic spacing, subpopulation
+mode) for the process.
 """
 from __future__ import annotations
 
@@ -32,7 +33,8 @@ _cache: Dict[tuple, Tuple[int, int]] = {}
 
 def _key(batch: Batch, mode: str) -> tuple:
     shapes = tuple(sorted({(t.n_thick, t.n_crowns, t.n_thin, t.n_sites) for t in batch.pack.topos}))
-    return shapes, mode, bool(batch.is_dynamic.any())
+    pops = None if batch.pops is None else (batch.pops.mode, batch.pops.K, batch.pops.flags)
+    return shapes, mode, bool(batch.is_dynamic.any()), pops
 
 
 def feasible_blocks(batch: Batch, mode: str) -> list:
@@ -47,6 +49,11 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
         L.cfg = SolverConfig(mode=mode).ctype()
         L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if batch.is_dynamic.any() else 0
         L.lattices_host = C.addressof(batch.pack.desc_host)
+        L.n_pops = 1
+        if batch.pops is not None:     # explicit populations widen the LDS TM rows
+            L.n_pops = batch.pops.K
+            L.pop_mode = _lib.MF_POP_MEAN_FIELD if batch.pops.mode == "mean_field" else _lib.MF_POP_EXPLICIT
+            L.pop_flags = batch.pops.flags
         lds = C.c_int64(0)
         if lib.mf_run_lds_bytes(C.byref(L), C.byref(lds)) == 0:
             out.append(blk)
@@ -82,7 +89,8 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_step
         for ncg in n_cg:
             cfg = SolverConfig(mode=mode, n_cg=ncg)
             trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
-                          batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn)
+                          batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn,
+                          pops=batch.pops.subset(order) if batch.pops is not None else None)
             trial.block = blk
             trial.run(W, t0=0, cfg=cfg)                      # equilibrate + warm-up
             s = torch.cuda.current_stream()

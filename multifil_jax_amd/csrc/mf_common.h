@@ -25,10 +25,17 @@ constexpr int WAVE = 64;
 
 // Per-instance table layout (floats).
 constexpr int TAB_HDR = 16;                 // pCa, d_kin, d_abs, valid, ...
-constexpr int TAB_TM = TAB_HDR;             // 27 configs x 4 rows x 4 cumsums
-constexpr int TAB_XB = TAB_TM + 27 * 16;    // 2*nb keys x 6 rows x 8 (6 cumsums + pad)
-__host__ __device__ inline int64_t tab_abs(int nb) { return TAB_XB + (int64_t)2 * nb * 48; }
-__host__ __device__ inline int64_t tab_floats(int nb) { return tab_abs(nb) + (int64_t)2 * nb * 4; }
+constexpr int TAB_TM = TAB_HDR;             // ktm x 27 configs x 4 rows x 4 cumsums
+constexpr int TAB_XB = TAB_TM + 27 * 16;    // (ktm = 1) kxb x 2*nb keys x 6 rows x 8 (6 cumsums + pad)
+// ktm / kxb: TM / XB table sets, one per population of an explicit
+// subpopulation run (labels select the set), else 1.
+__host__ __device__ inline int64_t tab_xb(int ktm) { return TAB_TM + (int64_t)ktm * 27 * 16; }
+__host__ __device__ inline int64_t tab_abs(int nb, int ktm = 1, int kxb = 1) {
+  return tab_xb(ktm) + (int64_t)kxb * 2 * nb * 48;
+}
+__host__ __device__ inline int64_t tab_floats(int nb, int ktm = 1, int kxb = 1) {
+  return tab_abs(nb, ktm, kxb) + (int64_t)kxb * 2 * nb * 4;
+}
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11)
@@ -136,8 +143,10 @@ __device__ __forceinline__ T kT_pNnm(T temp_c) {
 
 // 4x4 TM generator of neighbour configuration cfg = 9 n2 + 3 n3 + n_closed
 // (transitions.py:434-513, rate_functions.py:165-311).
-template <typename T>
-__device__ inline void tm_generator(const float* prm, T pCa, int cfg, T Q[4][4]) {
+// ACC: Q += w * generator (the mean-field blend of populations,
+// transitions.py:585-597), else Q = generator.
+template <typename T, bool ACC = false>
+__device__ inline void tm_generator(const float* prm, T pCa, int cfg, T Q[4][4], T w = T(1)) {
   const T n2 = (T)(cfg / 9), n3 = (T)((cfg / 3) % 3), nc = (T)(cfg % 3);
   const T JC = prm[P_TM_JC], JM = prm[P_TM_JM];
   const T h = JC * n2 + JM * n3 - T(0.5) * (JC + JM) * nc;
@@ -150,21 +159,24 @@ __device__ inline void tm_generator(const float* prm, T pCa, int cfg, T Q[4][4])
   const T k10 = (T(prm[P_TM_K01]) / T(prm[P_TM_KEQ01])) * bwd;
   const T k21 = (T(prm[P_TM_K12]) / T(prm[P_TM_KEQ12])) * bwd;
   const T k32 = (T(prm[P_TM_K23]) / T(prm[P_TM_KEQ23])) * bwd;
+  if (!ACC) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) Q[i][j] = T(0);
-  Q[0][0] = -k01; Q[0][1] = k01;
-  Q[1][0] = k10; Q[1][1] = -(k10 + k12); Q[1][2] = k12;
-  Q[2][1] = k21; Q[2][2] = -(k21 + k23); Q[2][3] = k23;
-  Q[3][0] = k30; Q[3][2] = k32; Q[3][3] = -(k30 + k32);
+      for (int j = 0; j < 4; ++j) Q[i][j] = T(0);
+  }
+  auto put = [&](int i, int j, T v) { if (ACC) Q[i][j] += w * v; else Q[i][j] = v; };
+  put(0, 0, -k01); put(0, 1, k01);
+  put(1, 0, k10); put(1, 1, -(k10 + k12)); put(1, 2, k12);
+  put(2, 1, k21); put(2, 2, -(k21 + k23)); put(2, 3, k23);
+  put(3, 0, k30); put(3, 2, k32); put(3, 3, -(k30 + k32));
 }
 
 // 6x6 XB generator at axial offset x, spacing d, permissiveness perm
 // (transitions.py:632-804, rate_functions.py:318-753).
-template <typename T>
+template <typename T, bool ACC = false>
 __device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, bool absorbing,
-                                    T Q[6][6]) {
+                                    T Q[6][6], T w = T(1)) {
   const T cap = T(RATE_CAP);
   const T r = sqrt(x * x + d * d);
   const T th = atan2(d, x);
@@ -194,18 +206,21 @@ __device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, b
   const T cab = pow(ca, b);
   const T r50 = T(prm[P_SRX_K0]) +
                 ((T(prm[P_SRX_KMAX]) - T(prm[P_SRX_K0])) * cab) / (pow(T(prm[P_SRX_CA50]), b) + cab);
+  if (!ACC) {
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) Q[i][j] = T(0);
-  if (!absorbing) {  // rows 0 and 4 zeroed in the absorbing variant (transitions.py:1077)
-    Q[0][0] = -(r01 + r04 + r05); Q[0][1] = r01; Q[0][4] = r04; Q[0][5] = r05;
-    Q[4][0] = r40; Q[4][4] = -r40;
+      for (int j = 0; j < 6; ++j) Q[i][j] = T(0);
   }
-  Q[1][0] = r10; Q[1][1] = -(r10 + r12); Q[1][2] = r12;
-  Q[2][1] = r21; Q[2][2] = -(r21 + r23); Q[2][3] = r23;
-  Q[3][2] = r32; Q[3][3] = -(r32 + r34); Q[3][4] = r34;
-  Q[5][0] = r50; Q[5][5] = -r50;
+  auto put = [&](int i, int j, T v) { if (ACC) Q[i][j] += w * v; else Q[i][j] = v; };
+  if (!absorbing) {  // rows 0 and 4 zeroed in the absorbing variant (transitions.py:1077)
+    put(0, 0, -(r01 + r04 + r05)); put(0, 1, r01); put(0, 4, r04); put(0, 5, r05);
+    put(4, 0, r40); put(4, 4, -r40);
+  }
+  put(1, 0, r10); put(1, 1, -(r10 + r12)); put(1, 2, r12);
+  put(2, 1, r21); put(2, 2, -(r21 + r23)); put(2, 3, r23);
+  put(3, 2, r32); put(3, 3, -(r32 + r34)); put(3, 4, r34);
+  put(5, 0, r50); put(5, 5, -r50);
 }
 
 // ---------------------------------------------------------------------------

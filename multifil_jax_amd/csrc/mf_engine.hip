@@ -65,32 +65,76 @@ __host__ __device__ inline StateLayout state_layout(int Tk, int nc, int Tn, int 
 // ---------------------------------------------------------------------------
 // transition tables: one matrix per thread
 // ---------------------------------------------------------------------------
-// TM row block `cfg` (27 neighbour configurations): FP64 expm of Q dt, stored
-// as FP32 cumulative rows (transitions.py:434-513, 99-190).
-__device__ __forceinline__ void tm_table_entry(const float* prm, double pCa, double dt, int cfg, float* tab) {
+// The populations of one instance (core/subpopulation.py).  Without
+// subpopulations K = 1 and row 0 is the instance's parameter row.
+struct Pops {
+  const float* prm;   // [K][MF_N_PARAMS], row 0 = wild type
+  const float* frac;  // [K] mean-field weights
+  int K;
+  bool blend_tm, blend_xb;  // mean field: generator = sum_k f_k Q_k
+};
+
+__device__ __forceinline__ Pops pops_of(const mf_launch& L, int b) {
+  Pops p;
+  p.K = L.pop_mode != MF_POP_NONE ? L.n_pops : 1;
+  p.prm = p.K > 1 ? L.pop_params + (int64_t)b * p.K * MF_N_PARAMS : L.params + (int64_t)b * MF_N_PARAMS;
+  p.frac = p.K > 1 ? L.pop_fractions + (int64_t)b * p.K : nullptr;
+  const bool mf = L.pop_mode == MF_POP_MEAN_FIELD && p.K > 1;
+  p.blend_tm = mf && (L.pop_flags & MF_POP_TM);
+  p.blend_xb = mf && (L.pop_flags & MF_POP_XB);
+  return p;
+}
+
+// TM row block `cfg` (27 neighbour configurations) of table set `set`: FP64
+// expm of Q dt, stored as FP32 cumulative rows (transitions.py:434-513,
+// 99-190, 580-602).
+__device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pCa, double dt, int cfg,
+                                               float* tab) {
   double Q[4][4];
-  tm_generator<double>(prm, pCa, cfg, Q);
+  if (P.blend_tm) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Q[i][j] = 0.0;
+    for (int k = 0; k < P.K; ++k)
+      tm_generator<double, true>(P.prm + k * MF_N_PARAMS, pCa, cfg, Q, (double)P.frac[k]);
+  } else {
+    tm_generator<double>(P.prm + set * MF_N_PARAMS, pCa, cfg, Q);
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
   expm_ref<4, double>(Q);
+  float* row = tab + TAB_TM + (set * 27 + cfg) * 16;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double c = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) { c += Q[i][j]; tab[TAB_TM + cfg * 16 + i * 4 + j] = (float)c; }
+    for (int j = 0; j < 4; ++j) { c += Q[i][j]; row[i * 4 + j] = (float)c; }
   }
 }
 
-// XB key (perm * nb + bin) at spacing d: the 6 cumulative rows of the
-// sampling table (transitions.py:632-922), or, for the absorbing chain
-// (rows 0 and 4 zeroed, transitions.py:1029-1078), the three entries the
-// metrics use.
-__device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, double d, double dt, double center,
-                                      int key, int nb, bool absorbing, float* xbrows, float* absrows) {
+// XB key (set * 2 nb + perm * nb + bin) at spacing d: the 6 cumulative rows
+// of the sampling table (transitions.py:632-922), or, for the absorbing
+// chain (rows 0 and 4 zeroed, transitions.py:1029-1078), the three entries
+// the metrics use.  Mean-field populations blend their generators
+// (transitions.py:925-975).
+__device__ __forceinline__ void xb_table_entry(const Pops& P, double pCa, double d, double dt, double center,
+                                               int key, int nb, bool absorbing, float* xbrows, float* absrows) {
   double Q[6][6];
-  xb_generator<double>(prm, pCa, d, center, (double)(key / nb), absorbing, Q);
+  const int set = key / (2 * nb), k2 = key % (2 * nb);
+  const double perm = (double)(k2 / nb);
+  if (P.blend_xb) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) Q[i][j] = 0.0;
+    for (int k = 0; k < P.K; ++k)
+      xb_generator<double, true>(P.prm + k * MF_N_PARAMS, pCa, d, center, perm, absorbing, Q, (double)P.frac[k]);
+  } else {
+    xb_generator<double>(P.prm + set * MF_N_PARAMS, pCa, d, center, perm, absorbing, Q);
+  }
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -115,20 +159,27 @@ __device__ __forceinline__ void xb_table_entry(const float* prm, double pCa, dou
   }
 }
 
-// All 2*nb XB (or absorbing) entries of one instance at spacing d, strided
-// over a workgroup.  Kept out of line: the FP64 6x6 expm needs far more
-// registers than the step kernel around it may hold.
-__device__ __attribute__((noinline)) void xb_table_rebuild(const float* prm, float pCa, double d, float dt,
-                                                           const float* centers, int nb, bool absorbing,
-                                                           float* xbrows, float* absrows, int tid,
-                                                           int nthreads) {
-  for (int key = tid; key < 2 * nb; key += nthreads)
-    xb_table_entry(prm, (double)pCa, (double)d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
+// All kxb * 2 nb XB (or absorbing) entries of one instance at spacing d,
+// strided over a workgroup.  Kept out of line: the FP64 6x6 expm needs far
+// more registers than the step kernel around it may hold.
+__device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float pCa, double d, float dt,
+                                                           const float* centers, int nb, int kxb,
+                                                           bool absorbing, float* xbrows, float* absrows,
+                                                           int tid, int nthreads) {
+  for (int key = tid; key < kxb * 2 * nb; key += nthreads)
+    xb_table_entry(P, (double)pCa, (double)d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
                    xbrows, absrows);
 }
 
+// Launch shape: B instances x (ktm*27 + 2 kxb 2 nb_max) entries.
+__host__ __device__ inline int table_sets(const mf_launch& L, int flag) {
+  return (L.pop_mode == MF_POP_EXPLICIT && (L.pop_flags & flag)) ? L.n_pops : 1;
+}
+
 __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_max) {
-  const int per = 27 + 4 * nb_max;
+  const int ktm = table_sets(L, MF_POP_TM), kxb = table_sets(L, MF_POP_XB);
+  const int nxb = kxb * 2 * nb_max;
+  const int per = ktm * 27 + 2 * nxb;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int b = (int)(gid / per);
   const int m = (int)(gid % per);
@@ -136,7 +187,7 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   const mf_instance I = L.instances[b];
   const mf_lattice G = L.lattices[I.lattice];
   const int nb = G.n_bins;
-  const float* prm = L.params + (int64_t)b * MF_N_PARAMS;
+  const Pops P = pops_of(L, b);
   const float* drv = L.drivers + ((int64_t)b * L.t_stride + I.t0 + L.t_base) * 3;
   const float pCa = drv[1];
   float d = drv[2];
@@ -148,17 +199,20 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   float* tab = L.tables + I.table_off;
   if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
   const double dt = L.cfg.dt;
-  if (m < 27) {
-    tm_table_entry(prm, (double)pCa, dt, m, tab);
+  if (m < ktm * 27) {
+    tm_table_entry(P, m / 27, (double)pCa, dt, m % 27, tab);
     return;
   }
-  int key = m - 27;
-  const bool absorbing = key >= 2 * nb_max;
-  if (absorbing) key -= 2 * nb_max;
-  if (key >= 2 * nb) return;
+  int key = m - ktm * 27;
+  const bool absorbing = key >= nxb;
+  if (absorbing) key -= nxb;
+  // key = set * 2 nb_max + k2 over the launch's largest bin count
+  const int set = key / (2 * nb_max), k2 = key % (2 * nb_max);
+  if (k2 >= 2 * nb) return;
+  key = set * 2 * nb + k2;
   const float* centers = L.lat_f32 + G.bin_centers;
-  xb_table_entry(prm, (double)pCa, (double)d, dt, (double)centers[key % nb], key, nb, absorbing, tab + TAB_XB,
-                 tab + tab_abs(nb));
+  xb_table_entry(P, (double)pCa, (double)d, dt, (double)centers[k2 % nb], key, nb, absorbing,
+                 tab + tab_xb(ktm), tab + tab_abs(nb, ktm, kxb));
 }
 
 // ---------------------------------------------------------------------------
@@ -167,6 +221,7 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
 struct Dims {
   int Tk, nc, Tn, ns, nx, W, nf, nb;
   int nk, N, X, S, nch;
+  int ktm, kxb;  // TM / XB table sets (explicit subpopulations), else 1
 };
 
 // Per-lane node slots (chain-major).
@@ -243,7 +298,7 @@ struct Shared {
   Big<short> fcount;     // [Tn*nf]
   Big<signed char> tprev;  // [S] same-chain neighbours as offsets along the filament
   Big<signed char> tnext;  // [S] (0: none, the site references itself)
-  LDS float* tmcum;      // [27*16]
+  LDS float* tmcum;      // [ktm*27*16]
   LDS float* edges;      // [nb+1]
   LDS float* prm;        // [MF_N_PARAMS]
   LDS float* red;        // [8][NW] scalar reduction ring
@@ -289,7 +344,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.fcount = big(2LL * d.Tn * d.nf);
   L.tprev = big(d.S);
   L.tnext = big(d.S);
-  L.tmcum = take(4 * 27 * 16);
+  L.tmcum = take(4 * 27 * 16 * d.ktm);
   L.edges = take(4LL * (d.nb + 1));
   L.prm = take(4 * MF_N_PARAMS);
   L.red = take(4 * 8 * NW);
@@ -1048,6 +1103,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   dm.W = G.width; dm.nf = G.n_faces; dm.nb = G.n_bins;
   dm.nk = dm.Tk * dm.nc; dm.N = dm.nk + dm.Tn * dm.ns; dm.X = dm.nk * dm.nx; dm.S = dm.Tn * dm.ns;
   dm.nch = dm.Tk + dm.Tn;
+  dm.ktm = table_sets(L, MF_POP_TM);
+  dm.kxb = table_sets(L, MF_POP_XB);
   // GW variants: this instance's slice of the device workspace
   char* ws = GW ? (char*)L.workspace + (int64_t)b * L.workspace_stride : nullptr;
   Shared<REF64, GW> sh = carve<BS, REF64, GW>((LDS char*)smem, ws, dm);
@@ -1065,7 +1122,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   // ---- static per-workgroup data ------------------------------------------
   for (int i = tid; i < MF_N_PARAMS; i += BS) sh.prm[i] = prm_g[i];
   for (int i = tid; i <= nb; i += BS) sh.edges[i] = lf[G.bin_edges + i];
-  for (int i = tid; i < 27 * 16; i += BS) sh.tmcum[i] = tab[TAB_TM + i];
+  for (int i = tid; i < dm.ktm * 27 * 16; i += BS) sh.tmcum[i] = tab[TAB_TM + i];
   for (int i = tid; i < dm.Tn * dm.nf * dm.W; i += BS) sh.fsites[i] = (short)li[G.face_sites + i];
   for (int i = tid; i < dm.Tn * dm.nf; i += BS) sh.fcount[i] = (short)li[G.face_count + i];
   for (int i = tid; i < S; i += BS) {
@@ -1182,14 +1239,18 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   // holds them (the host sets LAUNCH_TABLES_IN_LDS and sizes the dynamic
   // LDS), else read from the instance's table in device memory.  Read
   // through generic pointers, which address either.
-  float* xbtab = tab + TAB_XB;
-  float* abstab = tab + tab_abs(nb);
+  float* xbtab = tab + tab_xb(dm.ktm);
+  float* abstab = tab + tab_abs(nb, dm.ktm, dm.kxb);
+  // explicit subpopulations: per-head and per-site population labels
+  const uint8_t* xlab = dm.kxb > 1 ? L.pop_labels + I.label_off : nullptr;
+  const uint8_t* tlab = dm.ktm > 1 ? L.pop_labels + I.label_off + X : nullptr;
+  const Pops pops = pops_of(L, b);
   if (L.launch_flags & LAUNCH_TABLES_IN_LDS) {
     LDS float* tl = (LDS float*)(smem + lds_offsets<BS, REF64, GW>(dm).total);
-    const int nt = 2 * nb * 48 + 2 * nb * 4;
+    const int nt = dm.kxb * (2 * nb * 48 + 2 * nb * 4);
     for (int i = tid; i < nt; i += BS) tl[i] = tab[TAB_XB + i];
     xbtab = (float*)tl;
-    abstab = (float*)(tl + 2 * nb * 48);
+    abstab = (float*)(tl + dm.kxb * 2 * nb * 48);
     __syncthreads();
   }
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
@@ -1210,7 +1271,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         int nr; float xd;
         const int slot = sh.xslot[j] & 0x7fff;
         nearest_site<REF64>(sh, dm, slot, slot / dm.nf, j / dm.nx, nr, xd);
-        sh.xkey[j] = (short)bin_of(sh.edges, nb, inv_w, xd);  // tm == 0 everywhere: permissiveness 0
+        // tm == 0 everywhere: permissiveness 0
+        sh.xkey[j] = (short)((xlab ? xlab[j] * 2 * nb : 0) + bin_of(sh.edges, nb, inv_w, xd));
       }
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
@@ -1222,7 +1284,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       if (DYN && rad.on) {
         // kinetics at the carried spacing (simulation.py:691-693): rebuild
         // the XB sampling rows; d_ref = ls (z0 / z)^nu (simulation.py:694-696)
-        xb_table_rebuild(prm_g, pCa, ls_carry, L.cfg.dt, centers, nb, false, xbtab, abstab, tid, BS);
+        xb_table_rebuild(pops, pCa, ls_carry, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid, BS);
         __syncthreads();
         rad.dref = lsd * powf(z_trace0 / zl, I.nu);
       }
@@ -1251,7 +1313,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         if (st == 3 && sh.tbound[s] >= 0) {
           ns = 3;  // locked open under a bound head: [0, 0, 0, 1]
         } else {
-          const LDS float* c = sh.tmcum + cfg * 16 + st * 4;
+          const LDS float* c = sh.tmcum + ((tlab ? tlab[s] * 27 : 0) + cfg) * 16 + st * 4;
           ns = u < c[0] ? 0 : u < c[1] ? 1 : u < c[2] ? 2 : u < c[3] ? 3 : 0;
         }
         tmn[s] = (unsigned char)ns;
@@ -1278,13 +1340,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           nearest_site<REF64>(sh, dm, slot, thin, j / dm.nx, q_near[q], xd);
           q_near[q] += thin * dm.ns;  // thin-global site index from here on
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
-          const float* r0 = xbtab + (int64_t)bn * 48 + old * 8;
-          const float* r1 = xbtab + (int64_t)(nb + bn) * 48 + old * 8;
+          const int lb = xlab ? xlab[j] * 2 * nb : 0;
+          const float* r0 = xbtab + (int64_t)(lb + bn) * 48 + old * 8;
+          const float* r1 = xbtab + (int64_t)(lb + nb + bn) * 48 + old * 8;
           q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
           q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
           if (!(DYN && rad.on) && (old == 2 || old == 3)) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
           if (DYN) { q_kold[q] = keyold; q_sold[q] = old; }
-          q_bin[q] = bn;
+          q_bin[q] = lb + bn;
           q_occ[q] = sh.tbound[q_near[q]] >= 0;
           q_old[q] = old;
         }
@@ -1379,7 +1442,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     if (DYN && rad.on) {
       // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
       // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
-      xb_table_rebuild(prm_g, pCa, ls_new, L.cfg.dt, centers, nb, true, xbtab, abstab, tid, BS);
+      xb_table_rebuild(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS);
       __syncthreads();
       float m_atpe = 0.0f, m_tear = 0.0f;
 #pragma unroll
@@ -1605,12 +1668,14 @@ const Variant* variants(int& n) {
   return v;
 }
 
-Dims dims_of(const mf_lattice& G) {
+Dims dims_of(const mf_lattice& G, const mf_launch& L) {
   Dims d;
   d.Tk = G.n_thick; d.nc = G.n_crowns; d.Tn = G.n_thin; d.ns = G.n_sites; d.nx = G.n_xb_per_crown;
   d.W = G.width; d.nf = G.n_faces; d.nb = G.n_bins;
   d.nk = d.Tk * d.nc; d.N = d.nk + d.Tn * d.ns; d.X = d.nk * d.nx; d.S = d.Tn * d.ns;
   d.nch = d.Tk + d.Tn;
+  d.ktm = table_sets(L, MF_POP_TM);
+  d.kxb = table_sets(L, MF_POP_XB);
   return d;
 }
 
@@ -1640,7 +1705,7 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
     bool ok = true;
     int64_t need = 0, wneed = 0;
     for (const auto& G : lats) {
-      const Dims d = dims_of(G);
+      const Dims d = dims_of(G, *L);
       int ms, mx;
       needs(d, v[i].bs, ms, mx);
       int nchw = (d.nch + v[i].bs / WAVE - 1) / (v[i].bs / WAVE);
@@ -1675,6 +1740,15 @@ int fetch_lattices(const mf_launch* L, std::vector<mf_lattice>& lats) {
     if (G.n_bins <= 0 || G.n_bins > MF_MAX_BINS) return fail("n_bins out of range");
     if (G.n_xb_per_crown <= 0 || G.width <= 0) return fail("bad lattice");
   }
+  if (L->pop_mode != MF_POP_NONE) {
+    if (L->pop_mode != MF_POP_MEAN_FIELD && L->pop_mode != MF_POP_EXPLICIT) return fail("unknown pop_mode");
+    if (L->n_pops < 1 || L->n_pops > MF_MAX_POPS) return fail("n_pops out of range");
+    if (L->n_pops > 1 && !L->pop_params) return fail("subpopulations need pop_params");
+    if (L->pop_mode == MF_POP_MEAN_FIELD && L->n_pops > 1 && !L->pop_fractions)
+      return fail("mean-field subpopulations need pop_fractions");
+    if (L->pop_mode == MF_POP_EXPLICIT && (L->pop_flags & (MF_POP_XB | MF_POP_TM)) && !L->pop_labels)
+      return fail("explicit subpopulations need pop_labels");
+  }
   return 0;
 }
 
@@ -1690,6 +1764,10 @@ int64_t mf_state_bytes(int32_t n_thick, int32_t n_crowns, int32_t n_thin, int32_
   return state_layout(n_thick, n_crowns, n_thin, n_sites, n_xb_per_crown).bytes;
 }
 int64_t mf_table_floats(int32_t n_bins) { return tab_floats(n_bins); }
+int64_t mf_table_floats_pop(int32_t n_bins, int32_t n_pop_tm, int32_t n_pop_xb) {
+  if (n_pop_tm < 1 || n_pop_xb < 1 || n_pop_tm > MF_MAX_POPS || n_pop_xb > MF_MAX_POPS) return -1;
+  return tab_floats(n_bins, n_pop_tm, n_pop_xb);
+}
 
 int mf_tables(const mf_launch* L, void* stream) {
   if (!L) return fail("null launch");
@@ -1697,7 +1775,8 @@ int mf_tables(const mf_launch* L, void* stream) {
   if (int e = fetch_lattices(L, lats)) return e;
   int nb_max = 0;
   for (const auto& G : lats) nb_max = std::max(nb_max, G.n_bins);
-  const int64_t total = (int64_t)L->B * (27 + 4 * nb_max);
+  const int64_t total =
+      (int64_t)L->B * (table_sets(*L, MF_POP_TM) * 27 + 4 * nb_max * table_sets(*L, MF_POP_XB));
   const int grid = (int)((total + 255) / 256);
   hipLaunchKernelGGL(tables_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, *L, nb_max);
   HIPCHK(hipGetLastError());
@@ -1736,7 +1815,9 @@ int mf_run(const mf_launch* L, void* stream) {
                 " bytes per instance (mf_workspace_bytes)");
   // transition rows into LDS when they fit beside the instance arrays
   int64_t tab_bytes = 0;
-  for (const auto& G : lats) tab_bytes = std::max<int64_t>(tab_bytes, 4LL * (2LL * G.n_bins * 48 + 2LL * G.n_bins * 4));
+  const int kxb = table_sets(*L, MF_POP_XB);
+  for (const auto& G : lats)
+    tab_bytes = std::max<int64_t>(tab_bytes, 4LL * kxb * (2LL * G.n_bins * 48 + 2LL * G.n_bins * 4));
   mf_launch Lk = *L;
   Lk.launch_flags &= ~LAUNCH_TABLES_IN_LDS;
   if (lds + tab_bytes <= 160 * 1024) {

@@ -153,6 +153,67 @@ def default_block(topos) -> int:
     return 768 if nch <= 12 else 1024
 
 
+class Populations:
+    """Subpopulations of a batch in the engine's form (core/subpopulation.py;
+    simulation.py:514-577, 645-677).
+
+    mode: "mean_field" (one blended generator per kinetics, sum_k f_k Q_k)
+        or "explicit" (every head / site carries a population label that
+        selects its own transition rows).
+    params: (B, K, 44) float32 population constants; row 0 of every
+        instance is its wild-type row (the batch's params row), rows k > 0
+        its scaled copies (only xb_* / tm_* fields may differ).
+    fractions: (B, K) float32 mean-field weights.
+    xb_labels: explicit: per instance a (X,) uint8 array of head labels;
+    tm_labels: explicit: per instance a (S,) uint8 array of site labels.
+    has_xb / has_tm: whether the populations change the XB / TM kinetics
+        (any xb_* / tm_* field scaled); a kinetics they leave alone runs
+        with the wild-type rows, as the reference passes subpop=None there.
+    """
+
+    def __init__(self, mode, params, fractions=None, xb_labels=None, tm_labels=None,
+                 has_xb=True, has_tm=False):
+        if mode not in ("mean_field", "explicit"):
+            raise ValueError(f"unknown subpopulation mode {mode!r}")
+        self.mode = mode
+        self.params = np.ascontiguousarray(params, dtype=np.float32)
+        if self.params.ndim != 3 or self.params.shape[2] != N_PARAM_FIELDS:
+            raise ValueError(f"population params must be (B, K, {N_PARAM_FIELDS})")
+        self.B, self.K = self.params.shape[:2]
+        if not 1 <= self.K <= _lib.MF_MAX_POPS:
+            raise ValueError(f"1..{_lib.MF_MAX_POPS} populations supported, got {self.K}")
+        self.fractions = (np.ascontiguousarray(fractions, dtype=np.float32) if fractions is not None
+                          else np.full((self.B, self.K), 1.0 / self.K, np.float32))
+        if self.fractions.shape != (self.B, self.K):
+            raise ValueError(f"fractions must be ({self.B}, {self.K})")
+        self.has_xb, self.has_tm = bool(has_xb), bool(has_tm)
+        self.xb_labels = xb_labels
+        self.tm_labels = tm_labels
+        if mode == "explicit":
+            for name, lab in (("xb_labels", xb_labels), ("tm_labels", tm_labels)):
+                if lab is None or len(lab) != self.B:
+                    raise ValueError(f"explicit subpopulations need {name} for every instance")
+                for a in lab:
+                    a = np.asarray(a)
+                    if a.size and (a.min() < 0 or a.max() >= self.K):
+                        raise ValueError(f"{name} must lie in [0, {self.K})")
+
+    @property
+    def flags(self) -> int:
+        return (_lib.MF_POP_XB if self.has_xb else 0) | (_lib.MF_POP_TM if self.has_tm else 0)
+
+    def table_sets(self):
+        """(TM, XB) transition-table sets per instance."""
+        ex = self.mode == "explicit"
+        return (self.K if ex and self.has_tm else 1), (self.K if ex and self.has_xb else 1)
+
+    def subset(self, idx) -> "Populations":
+        idx = np.asarray(idx)
+        pick = (lambda lab: [lab[i] for i in idx]) if self.mode == "explicit" else (lambda lab: None)
+        return Populations(self.mode, self.params[idx], self.fractions[idx],
+                           pick(self.xb_labels), pick(self.tm_labels), self.has_xb, self.has_tm)
+
+
 class Batch:
     """B instances resident on one GPU, advanced by persistent launches.
 
@@ -166,10 +227,12 @@ class Batch:
         fixed spacing.
     lengths: optional (B,) steps per instance (<= T); an instance stops
         after its own length, its later metric rows stay zero.
+    pops: optional Populations (subpopulations, one mode for the batch).
     """
 
     def __init__(self, pack: LatticePack, lattice_ids, params, drivers, seeds,
-                 dynamic=None, device: str = "cuda", want_metrics: bool = True, lengths=None):
+                 dynamic=None, device: str = "cuda", want_metrics: bool = True, lengths=None,
+                 pops: Optional[Populations] = None):
         torch = _torch()
         lib = _lib.load()
         self.lib = lib
@@ -208,14 +271,42 @@ class Batch:
             if got != lay["bytes"]:
                 raise RuntimeError("state layout mismatch between host and library")
         self.state_off = np.concatenate([[0], np.cumsum(sbytes)[:-1]]).astype(np.int64)
-        tfl = [int(lib.mf_table_floats(topos[i].n_xb_bins)) for i in self.lattice_ids]
+        self.pops = pops
+        ktm, kxb = pops.table_sets() if pops is not None else (1, 1)
+        if pops is not None:
+            if pops.B != self.B:
+                raise ValueError(f"populations cover {pops.B} instances, batch has {self.B}")
+            if not np.array_equal(pops.params[:, 0], params):
+                raise ValueError("population row 0 must equal the instance's parameter row")
+        tfl = [int(lib.mf_table_floats_pop(topos[i].n_xb_bins, ktm, kxb)) for i in self.lattice_ids]
         self.table_off = np.concatenate([[0], np.cumsum(tfl)[:-1]]).astype(np.int64)
+        self.label_off = np.zeros(self.B, np.int64)
+        labels = None
+        if pops is not None and pops.mode == "explicit":
+            chunks = []
+            for b, i in enumerate(self.lattice_ids):
+                t = topos[i]
+                X, S = t.total_xbs, t.n_thin * t.n_sites
+                xl = np.asarray(pops.xb_labels[b], np.uint8).ravel()
+                tl = np.asarray(pops.tm_labels[b], np.uint8).ravel()
+                if xl.size != X or tl.size != S:
+                    raise ValueError(f"instance {b}: labels must be ({X},) heads and ({S},) sites")
+                chunks += [xl, tl]
+            lens = np.array([c.size for c in chunks[0::2]]) + np.array([c.size for c in chunks[1::2]])
+            self.label_off = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+            labels = np.concatenate(chunks)
 
         dev = self.device
         self.d_lat = torch.from_numpy(pack.desc_bytes).to(dev)
         self.d_f32 = torch.from_numpy(pack.f32).to(dev)
         self.d_i32 = torch.from_numpy(pack.i32).to(dev)
         self.d_params = torch.from_numpy(params).to(dev)
+        self.d_pop_params = self.d_pop_frac = self.d_labels = None
+        if pops is not None:
+            self.d_pop_params = torch.from_numpy(pops.params).to(dev)
+            self.d_pop_frac = torch.from_numpy(pops.fractions).to(dev)
+            if labels is not None:
+                self.d_labels = torch.from_numpy(labels).to(dev)
         self.d_drivers = torch.from_numpy(drivers).to(dev)
         self.d_state = torch.zeros(int(sum(sbytes)), dtype=torch.uint8, device=dev)
         self.d_tables = torch.zeros(int(sum(tfl)), dtype=torch.float32, device=dev)
@@ -251,6 +342,7 @@ class Batch:
                 arr[b].seed = int(self.seeds[b])
                 arr[b].state_off = int(self.state_off[b])
                 arr[b].table_off = int(self.table_off[b])
+                arr[b].label_off = int(self.label_off[b])
             buf = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
             hit = _torch().from_numpy(buf).to(self.device)
             self._inst_cache[key] = hit
@@ -277,6 +369,14 @@ class Batch:
         L.block = int(self.block)
         L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if self.is_dynamic.any() else 0
         L.lattices_host = C.addressof(self.pack.desc_host)
+        L.n_pops = 1
+        if self.pops is not None:
+            L.n_pops = self.pops.K
+            L.pop_mode = _lib.MF_POP_MEAN_FIELD if self.pops.mode == "mean_field" else _lib.MF_POP_EXPLICIT
+            L.pop_flags = self.pops.flags
+            L.pop_params = self.d_pop_params.data_ptr()
+            L.pop_fractions = self.d_pop_frac.data_ptr()
+            L.pop_labels = self.d_labels.data_ptr() if self.d_labels is not None else None
         return L
 
     def _workspace(self, L):

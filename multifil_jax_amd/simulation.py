@@ -243,6 +243,7 @@ class SweepPlan:
         self.coords = coords
         self.dynamic = dynamic
         self.replicates = replicates
+        self.subpop = None       # subpopulation.ResolvedPopulations, when given
 
     @property
     def batch(self) -> int:
@@ -251,7 +252,7 @@ class SweepPlan:
 
 def plan_sweep(topology, duration_ms=1000.0, dt=1.0, pCa=4.5, z_line=900.0,
                lattice_spacing=14.0, K_lat=None, nu=0.0, dynamic_params=None,
-               replicates=1) -> SweepPlan:
+               replicates=1, subpopulation=None) -> SweepPlan:
     """Build the Cartesian sweep exactly as run() does (simulation.py:874-1022)."""
     n_steps = int(duration_ms / dt)
     if n_steps < 1:
@@ -319,6 +320,14 @@ def plan_sweep(topology, duration_ms=1000.0, dt=1.0, pCa=4.5, z_line=900.0,
                 sweep_axes.append((name, arr.tolist()))
                 param_tables[name] = arr.astype(np.float32)
                 param_axis[name] = name
+    # a subpopulation list is a candidate-style axis too (simulation.py:945-953)
+    if isinstance(subpopulation, list):
+        if isinstance(dynamic_params, list):
+            raise ValueError("Cannot sweep a dynamic_params list and a subpopulation list "
+                             "simultaneously (both are candidate-style axes).")
+        if not subpopulation:
+            raise ValueError("subpopulation list is empty")
+        sweep_axes.append(('subpopulation', list(range(len(subpopulation)))))
 
     if sweep_axes:
         grids = np.meshgrid(*[np.arange(len(c)) for _, c in sweep_axes], indexing='ij')
@@ -364,8 +373,26 @@ def plan_sweep(topology, duration_ms=1000.0, dt=1.0, pCa=4.5, z_line=900.0,
 
     params = np.stack([param(n) for n in DYNAMIC_FIELDS], axis=1).astype(np.float32)
     drivers = np.stack([z_b, pCa_b, ls_b], axis=-1).astype(np.float32)
-    return SweepPlan(n_steps, np.ascontiguousarray(drivers), params, K_b, nu_b, grid_shape,
+    plan = SweepPlan(n_steps, np.ascontiguousarray(drivers), params, K_b, nu_b, grid_shape,
                      axis_names, coords, dynamic, replicates)
+    if subpopulation is not None:
+        from .subpopulation import resolve
+        idx = flat['subpopulation'] if isinstance(subpopulation, list) else None
+        plan.subpop = resolve(subpopulation, topology, total, idx)
+    return plan
+
+
+def _populations(plan: SweepPlan, Populations):
+    """The engine's form of a resolved subpopulation: population rows,
+    fractions and, for explicit modes, per-simulation labels."""
+    from .subpopulation import population_params
+    rp = plan.subpop
+    prm = population_params(plan.params, rp)
+    xl = tl = None
+    if rp.mode == "explicit":
+        xl = [m.astype(np.uint8) for m in rp.xb_mask]
+        tl = [m.astype(np.uint8) for m in rp.tm_mask]
+    return Populations(rp.mode, prm, rp.fractions, xl, tl, has_xb=rp.has_xb, has_tm=rp.has_tm)
 
 
 # ============================================================================
@@ -391,12 +418,11 @@ def run(topology, duration_ms: float = 1000.0, dt: float = 1.0,
     launch; ``minibatch_size`` (an int) splits the batch into chunks to bound
     device memory, "auto" / None run everything at once.
     """
-    if subpopulation is not None:
-        raise NotImplementedError("subpopulation modes are not implemented by this engine")
-    from .engine import LatticePack, Batch, SolverConfig   # loads the device library
+    from .engine import LatticePack, Batch, SolverConfig, Populations   # loads the device library
     static_params = static_params or StaticParams()
     plan = plan_sweep(topology, duration_ms, dt, pCa, z_line, lattice_spacing, K_lat, nu,
-                      dynamic_params, replicates)
+                      dynamic_params, replicates, subpopulation)
+    pops = _populations(plan, Populations) if plan.subpop is not None else None
     B, T = plan.batch, plan.n_steps
     cfg = SolverConfig(mode=solver, n_newton=static_params.n_newton_steps,
                        n_cg=static_params.n_cg_steps, dt=dt)
@@ -411,7 +437,8 @@ def run(topology, duration_ms: float = 1000.0, dt: float = 1.0,
         s1 = min(B, s0 + chunk)
         bt = Batch(pack, np.zeros(s1 - s0, np.int32), plan.params[s0:s1], plan.drivers[s0:s1],
                    seeds=seeds[s0:s1],
-                   dynamic=(plan.K_lat[s0:s1], plan.nu[s0:s1]) if plan.dynamic else None)
+                   dynamic=(plan.K_lat[s0:s1], plan.nu[s0:s1]) if plan.dynamic else None,
+                   pops=pops.subset(np.arange(s0, s1)) if pops is not None else None)
         if block:
             bt.block = int(block)
         bt.run(T, cfg=cfg)

@@ -209,15 +209,60 @@ class Tables:
     P_abs: np.ndarray    # (2*nb, 6, 6)
 
 
-def make_tables(p, pCa, d, dt, bin_centers, d_abs=None) -> Tables:
+@dataclass
+class Pop:
+    """Subpopulations of one instance (core/subpopulation.py).
+
+    mode 'mean_field': one generator sum_k f_k Q_k (transitions.py:585-590,
+    966-969); 'explicit': per-population tables stacked K deep, selected by
+    the per-site / per-head labels (transitions.py:591-602, 971-975).
+    params: K parameter mappings (0 = wild type); fractions: K weights
+    (float32 values, as the device receives them); has_xb / has_tm: whether
+    the XB / TM kinetics are population-dependent at all.
+    """
+    mode: str
+    params: list
+    fractions: np.ndarray
+    has_xb: bool = True
+    has_tm: bool = False
+    xb_label: Optional[np.ndarray] = None   # (X,) explicit
+    tm_label: Optional[np.ndarray] = None   # (S,) explicit
+
+    def sets(self):
+        ex = self.mode == 'explicit'
+        return (len(self.params) if ex and self.has_tm else 1), (len(self.params) if ex and self.has_xb else 1)
+
+    def offsets(self, nb: int):
+        """Row offsets: TM config + 27 * label, XB key + 2 nb * label."""
+        ktm, kxb = self.sets()
+        tm = (27 * np.asarray(self.tm_label, np.int64)) if ktm > 1 else 0
+        xb = (2 * nb * np.asarray(self.xb_label, np.int64)) if kxb > 1 else 0
+        return tm, xb
+
+
+def _pop_generators(gen, pop: Optional[Pop], flag: bool, p):
+    """Generators of one kinetics: the wild type's, the mean-field blend, or
+    the K population blocks stacked along the first axis."""
+    if pop is None or not flag:
+        return gen(p)
+    Qs = [gen(pk) for pk in pop.params]
+    if pop.mode == 'mean_field':
+        f = np.asarray(pop.fractions, np.float32).astype(np.float64)
+        return sum(f[k] * Qs[k] for k in range(len(Qs)))
+    return np.concatenate(Qs)
+
+
+def make_tables(p, pCa, d, dt, bin_centers, d_abs=None, pop: Optional[Pop] = None) -> Tables:
     """expm(Q dt) for TM, XB and the absorbing XB generator (transitions.py:
     597-598, 993-1026, 1029-1078); ``d_abs`` is the spacing the metrics path
-    uses (simulation.py:711)."""
-    P_tm = expm_ref(tm_generators(p, pCa) * dt)
-    Q = xb_bin_generators(p, pCa, d, bin_centers)
+    uses (simulation.py:711); ``pop`` the subpopulations."""
+    has_tm = pop is not None and pop.has_tm
+    has_xb = pop is not None and pop.has_xb
+    P_tm = expm_ref(_pop_generators(lambda q: tm_generators(q, pCa), pop, has_tm, p) * dt)
+    Q = _pop_generators(lambda q: xb_bin_generators(q, pCa, d, bin_centers), pop, has_xb, p)
     P_xb = expm_ref(Q * dt)
     if d_abs is not None and float(d_abs) != float(d):
-        Q = xb_bin_generators(p, pCa, d_abs, bin_centers)
+        Q = _pop_generators(lambda q: xb_bin_generators(q, pCa, d_abs, bin_centers), pop, has_xb, p)
     P_abs = expm_ref(absorbing(Q) * dt)
     return Tables(P_tm, P_xb, P_abs)
 
@@ -335,9 +380,10 @@ def tm_configs(tm: np.ndarray, topo) -> np.ndarray:
     return (np.minimum(n2, 2) * 9 + np.minimum(n3, 2) * 3 + np.minimum(nc, 2)).reshape(-1)
 
 
-def tm_transition(tm, thin_bound, P_tm, u_tm, topo) -> np.ndarray:
-    """TM sampling with the bound-open lock (transitions.py:516-625)."""
-    cfg = tm_configs(tm, topo)
+def tm_transition(tm, thin_bound, P_tm, u_tm, topo, off=0) -> np.ndarray:
+    """TM sampling with the bound-open lock (transitions.py:516-625); ``off``
+    adds each site's population block (27 * label) to its row."""
+    cfg = tm_configs(tm, topo) + off
     s = tm.reshape(-1).astype(np.int64)
     prob = P_tm[cfg, s].copy()
     locked = (s == 3) & (thin_bound.reshape(-1) >= 0)
@@ -345,13 +391,14 @@ def tm_transition(tm, thin_bound, P_tm, u_tm, topo) -> np.ndarray:
     return sample_rows(prob, np.asarray(u_tm, np.float64)).astype(np.int8).reshape(tm.shape)
 
 
-def xb_transition(st: State, tm_new, P_xb, u_xb, topo):
-    """XB sampling and binding bookkeeping (transitions.py:1081-1245).
+def xb_transition(st: State, tm_new, P_xb, u_xb, topo, off=0):
+    """XB sampling and binding bookkeeping (transitions.py:1081-1245); ``off``
+    adds each head's population block (2 n_bins * label) to its key.
 
     Returns (xb_state, xb_bound, thin_bound) after the step.
     """
     ns = topo.n_sites
-    key = xb_keys(st.xb_xdist, st.xb_nearest, tm_new, topo)
+    key = xb_keys(st.xb_xdist, st.xb_nearest, tm_new, topo) + off
     old = st.xb_state.astype(np.int64)
     new = sample_rows(P_xb[key, old], np.asarray(u_xb, np.float64))
     perm = permissiveness(tm_new, st.xb_nearest, topo)
@@ -857,13 +904,14 @@ def solve(st: State, topo, p, z, d, cfg: SolverConfig, precond,
     return x, dd, it, res
 
 
-def kinetics(st: State, topo, tables: Tables, u_tm, u_xb) -> State:
+def kinetics(st: State, topo, tables: Tables, u_tm, u_xb, pop: Optional[Pop] = None) -> State:
     """kinetics_step (timestep.py:65-129): nearest sites, TM, XB."""
     s = st.copy()
+    off_tm, off_xb = pop.offsets(topo.n_xb_bins) if pop is not None else (0, 0)
     s.xb_nearest, s.xb_xdist = nearest_sites(s.thick.astype(np.float32),
                                              s.thin.astype(np.float32), topo)
-    tm_new = tm_transition(s.tm, s.thin_bound, tables.P_tm, u_tm, topo)
-    s.xb_state, s.xb_bound, s.thin_bound = xb_transition(s, tm_new, tables.P_xb, u_xb, topo)
+    tm_new = tm_transition(s.tm, s.thin_bound, tables.P_tm, u_tm, topo, off_tm)
+    s.xb_state, s.xb_bound, s.thin_bound = xb_transition(s, tm_new, tables.P_xb, u_xb, topo, off_xb)
     s.tm = tm_new
     return s
 
@@ -886,7 +934,7 @@ def init_state(topo, p, z0, cfg: SolverConfig, precond=None) -> State:
 
 
 def step(st: State, topo, p, tables: Tables, z, pCa, ls_drv, dz, u_tm, u_xb, cfg: SolverConfig,
-         precond, dynamic=None):
+         precond, dynamic=None, pop: Optional[Pop] = None):
     """One scan body (simulation.py:679-721).
 
     dynamic: None for fixed spacing, else (K_lat, d_ref).  Returns
@@ -895,9 +943,10 @@ def step(st: State, topo, p, tables: Tables, z, pCa, ls_drv, dz, u_tm, u_xb, cfg
     old = st.copy()
     old.thin = old.thin + float(np.float32(dz))
     pre_thick = old.thick.copy()
-    key_old = xb_keys(old.xb_xdist, old.xb_nearest, old.tm, topo)
+    key_old = xb_keys(old.xb_xdist, old.xb_nearest, old.tm, topo) + \
+        (pop.offsets(topo.n_xb_bins)[1] if pop is not None else 0)
     d_kin = old.ls if dynamic is not None else float(ls_drv)
-    s = kinetics(old, topo, tables, u_tm, u_xb)
+    s = kinetics(old, topo, tables, u_tm, u_xb, pop)
     if dynamic is None:
         x, d_new, it, res = solve(s, topo, p, z, d_kin, cfg, precond)
     else:
@@ -912,7 +961,7 @@ def step(st: State, topo, p, tables: Tables, z, pCa, ls_drv, dz, u_tm, u_xb, cfg
 
 
 def run(topo, p, z_trace, pCa_trace, ls_trace, dt, seed, cfg: SolverConfig,
-        K_lat=None, nu=0.0, n_steps=None, tables_fn=None):
+        K_lat=None, nu=0.0, n_steps=None, tables_fn=None, pop: Optional[Pop] = None):
     """run_single_sim for one instance (simulation.py:644-729), with the
     device's Philox uniforms.  Returns (final_state, metrics (T, 52))."""
     from .philox import uniforms, PURPOSE_TM, PURPOSE_XB
@@ -938,7 +987,7 @@ def run(topo, p, z_trace, pCa_trace, ls_trace, dt, seed, cfg: SolverConfig,
         if K_lat is None and key in cache:
             tab = cache[key]
         else:
-            tab = make_tables(p, pCa_trace[t], d_tab, dt, topo.xb_bin_centers)
+            tab = make_tables(p, pCa_trace[t], d_tab, dt, topo.xb_bin_centers, pop=pop)
             cache = {key: tab}
         if K_lat is not None:
             tab_m = None   # metrics tables use the solved spacing; patched below
@@ -946,14 +995,15 @@ def run(topo, p, z_trace, pCa_trace, ls_trace, dt, seed, cfg: SolverConfig,
         u_xb = uniforms(seed, t, PURPOSE_XB, X)
         if K_lat is None:
             st, mt = step(st, topo, p, tab, z_trace[t], pCa_trace[t], ls_trace[t], dz[t],
-                          u_tm, u_xb, cfg, pre)
+                          u_tm, u_xb, cfg, pre, pop=pop)
         else:
             st_prev = st
             st, mt = step(st, topo, p, tab, z_trace[t], pCa_trace[t], ls_trace[t], dz[t],
-                          u_tm, u_xb, cfg, pre, dynamic=dyn)
-            tab_m = make_tables(p, pCa_trace[t], st.ls, dt, topo.xb_bin_centers)
+                          u_tm, u_xb, cfg, pre, dynamic=dyn, pop=pop)
+            tab_m = make_tables(p, pCa_trace[t], st.ls, dt, topo.xb_bin_centers, pop=pop)
             old = st_prev.copy()
-            key_old = xb_keys(old.xb_xdist, old.xb_nearest, old.tm, topo)
+            key_old = xb_keys(old.xb_xdist, old.xb_nearest, old.tm, topo) + \
+                (pop.offsets(topo.n_xb_bins)[1] if pop is not None else 0)
             Pk = tab_m.P_abs[key_old]
             ox = old.xb_state.astype(np.int64)
             atp = float(np.sum(np.where(ox == 3, Pk[:, 3, 4], 0.0)))
