@@ -575,6 +575,62 @@ class SarcTopology:
             tm_prev_neighbor=prev, tm_next_neighbor=nxt,
             nrows=nrows, ncols=ncols, periodic=periodic)
 
+    # ------------------------------------------------------------------
+    # on-disk format: one .npz of the SoA arrays under the reference's field
+    # names, plus the integer counts and a format tag / version
+    FORMAT = "multifil-topology"
+    FORMAT_VERSION = 1
+
+    def save(self, path) -> None:
+        """Write the topology to ``path`` (.npz, no pickled objects)."""
+        meta = {k: np.int64(getattr(self, k)) for k in self.INT_FIELDS}
+        arrays = {k: np.asarray(getattr(self, k)) for k in self.ARRAY_FIELDS}
+        np.savez(path, format=np.array(self.FORMAT), format_version=np.int64(self.FORMAT_VERSION),
+                 nrows=np.int64(-1 if self.nrows is None else self.nrows),
+                 ncols=np.int64(-1 if self.ncols is None else self.ncols),
+                 periodic=np.bool_(self.periodic), **meta, **arrays)
+
+    @classmethod
+    def load(cls, path) -> 'SarcTopology':
+        """Read a topology written by ``save``; ValueError on another format,
+        a newer version, a missing field or inconsistent counts."""
+        with np.load(path, allow_pickle=False) as z:
+            if "format" not in z.files or str(z["format"]) != cls.FORMAT:
+                raise ValueError(f"{path}: not a {cls.FORMAT} file")
+            ver = int(z["format_version"])
+            if ver > cls.FORMAT_VERSION:
+                raise ValueError(f"{path}: format version {ver} is newer than {cls.FORMAT_VERSION}")
+            missing = [k for k in cls.INT_FIELDS + cls.ARRAY_FIELDS if k not in z.files]
+            if missing:
+                raise ValueError(f"{path}: missing fields {missing}")
+            kw = {k: int(z[k]) for k in cls.INT_FIELDS}
+            kw.update({k: np.array(z[k]) for k in cls.ARRAY_FIELDS})
+            nr, nc = int(z["nrows"]), int(z["ncols"])
+            kw.update(nrows=None if nr < 0 else nr, ncols=None if nc < 0 else nc,
+                      periodic=bool(z["periodic"]))
+        t = cls(**kw)
+        t._check()
+        return t
+
+    def _check(self):
+        """Shape and count consistency of the arrays the engine packs."""
+        X = self.n_thick * self.n_crowns * self.n_xb_per_crown
+        want = {
+            'crown_offsets': (self.n_thick, self.n_crowns), 'crown_rests': (self.n_thick, self.n_crowns),
+            'binding_offsets': (self.n_thin, self.n_sites), 'binding_rests': (self.n_thin, self.n_sites),
+            'tm_prev_neighbor': (self.n_thin, self.n_sites), 'tm_next_neighbor': (self.n_thin, self.n_sites),
+            'xb_to_thin_id': (X,), 'xb_to_thin_face': (X,), 'xb_valid': (X,),
+            'xb_to_site_indices': (X, self.max_sites_per_face),
+            'face_to_sites': (self.n_thin, self.n_faces_per_thin, self.max_sites_per_face),
+            'n_sites_per_face': (self.n_thin, self.n_faces_per_thin),
+            'xb_bin_edges': (self.n_xb_bins + 1,), 'xb_bin_centers': (self.n_xb_bins,),
+        }
+        if self.total_xbs != X:
+            raise ValueError(f"total_xbs {self.total_xbs} != n_thick * n_crowns * n_xb_per_crown = {X}")
+        for k, shape in want.items():
+            if tuple(np.shape(getattr(self, k))) != shape:
+                raise ValueError(f"{k}: shape {np.shape(getattr(self, k))}, expected {shape}")
+
     def valid_xb_targets(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Indices/targets of XBs with a real partner (sarc_geometry.py:285-316)."""
         idx = np.where(self.xb_valid)[0]

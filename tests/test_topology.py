@@ -116,3 +116,49 @@ def test_dynamic_params_api():
     assert float(w.pCa) == 5.0 and float(w.z_line) == 1000.0
     rows = DynamicParams(thick_k=[7000.0, 8000.0]).rows(2)
     assert rows.shape == (2, 44) and rows[1, 0] == 8000.0
+
+
+def test_save_load_round_trip(tmp_path, skeletal):
+    """The .npz format round-trips every field bit for bit, and a loaded
+    lattice packs to the same engine blobs."""
+    from multifil_jax_amd.params import get_lethocerus_params
+    from multifil_jax_amd.engine import LatticePack
+    s, d = skeletal
+    ls, ld = get_lethocerus_params()
+    for t in (SarcTopology.create(2, 2, s, d), SarcTopology.create(2, 2, ls, ld)):
+        path = tmp_path / "lattice.npz"
+        t.save(path)
+        u = SarcTopology.load(path)
+        for k in SarcTopology.INT_FIELDS:
+            assert getattr(u, k) == getattr(t, k)
+        for k in SarcTopology.ARRAY_FIELDS:
+            a, b = np.asarray(getattr(t, k)), np.asarray(getattr(u, k))
+            assert a.dtype == b.dtype and np.array_equal(a, b), k
+        assert (u.nrows, u.ncols, u.periodic) == (t.nrows, t.ncols, t.periodic)
+        p1, p2 = LatticePack([t]), LatticePack([u])
+        assert np.array_equal(p1.f32, p2.f32) and np.array_equal(p1.i32, p2.i32)
+        assert np.array_equal(p1.desc_bytes, p2.desc_bytes)
+
+
+def test_load_rejects_bad_files(tmp_path, skeletal):
+    s, d = skeletal
+    t = SarcTopology.create(2, 2, s, d)
+    bad = tmp_path / "other.npz"
+    np.savez(bad, x=np.zeros(3))
+    with pytest.raises(ValueError):
+        SarcTopology.load(bad)
+    good = tmp_path / "t.npz"
+    t.save(good)
+    with np.load(good) as z:
+        fields = {k: z[k] for k in z.files}
+    fields["format_version"] = np.int64(SarcTopology.FORMAT_VERSION + 1)
+    newer = tmp_path / "newer.npz"
+    np.savez(newer, **fields)
+    with pytest.raises(ValueError):
+        SarcTopology.load(newer)
+    fields["format_version"] = np.int64(1)
+    fields["crown_offsets"] = fields["crown_offsets"][:, :-1]
+    broken = tmp_path / "broken.npz"
+    np.savez(broken, **fields)
+    with pytest.raises(ValueError):
+        SarcTopology.load(broken)
