@@ -276,6 +276,7 @@ def workload_uneven(cfg, block=None, autotune=True):
                                    "dt 1 ms; default_rng(20261203)",
                        "solver": cfg.mode, "launches": 2,
                        "groups": [{"block": bt.block, "n_cg": getattr(bt, "tuned_cfg", cfg).n_cg,
+                                   "cg_forcing": getattr(bt, "tuned_cfg", cfg).cg_forcing,
                                    "instances": bt.B} for bt in batches]}}
 
 
@@ -381,7 +382,7 @@ def main():
         "config": {"workload": "batched sweep: 256 conditions on a 2x2 lattice, pCa U[4,7], "
                                "z_line U[850,1150] nm, dt 1 ms, seeds 0..255, fixed spacing 14 nm",
                    "lattice": "2x2", "batch": B, "solver": args.mode, "block": bt.block,
-                   "n_cg": cfg.n_cg, "autotuned": tuned is not None},
+                   "n_cg": cfg.n_cg, "cg_forcing": cfg.cg_forcing, "autotuned": tuned is not None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "run_kernel", "kernel_ms": ms_kernel, "launches": 1},

@@ -112,7 +112,10 @@ typedef struct mf_solver_cfg {
   float tol;           /* NaN: max(solver_tol, thick_k*1e-4, 0.25)         */
   float refine_tol;    /* pN, MF_SOLVER_REFINED stopping residual          */
   float dt;            /* ms                                               */
-  int32_t pad;
+  float cg_forcing;    /* MF_SOLVER_REFINED: CG stops at a relative r.u of
+                          max(cg_forcing refine_tol / max|F|,
+                          1e-5 max|F|)^2; 0 runs n_cg iterations (or to
+                          FP32 rounding)                                  */
 } mf_solver_cfg;
 
 typedef struct mf_launch {

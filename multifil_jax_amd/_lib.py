@@ -52,7 +52,7 @@ class MfInstance(C.Structure):
 class MfSolverCfg(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_newton", C.c_int32), ("n_cg", C.c_int32),
                 ("n_refine", C.c_int32), ("tol", C.c_float), ("refine_tol", C.c_float),
-                ("dt", C.c_float), ("pad", C.c_int32)]
+                ("dt", C.c_float), ("cg_forcing", C.c_float)]
 
 
 class MfLaunch(C.Structure):

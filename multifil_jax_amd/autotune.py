@@ -7,6 +7,9 @@ Two knobs decide the speed of a batch:
 * **CG iterations per Newton iteration** (refined mode only; reference mode
   keeps the reference's 6). Too few and Newton iterates more; too many and
   each iteration costs more.
+* **CG forcing term** (refined mode only): 0 runs the capped CG count, a
+  positive value stops CG once the correction is as accurate as the Newton
+  step can use.
 
 The tuner runs short trial launches of the same instances on a copy of the
 batch. It tries every block size with a kernel variant for these lattices,
@@ -27,8 +30,9 @@ from .engine import Batch, SolverConfig, default_block
 
 BLOCKS = (512, 768, 1024)
 N_CG = (4, 6, 8, 10, 12)
+CG_FORCING = (0.0, 0.003)
 
-_cache: Dict[tuple, Tuple[int, int]] = {}
+_cache: Dict[tuple, Tuple[int, int, float]] = {}
 
 
 def _key(batch: Batch, mode: str) -> tuple:
@@ -62,8 +66,8 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
 
 def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_steps: int = 20,
          n_cg: Sequence[int] = N_CG, blocks: Optional[Sequence[int]] = None, max_instances: int = 256,
-         verbose: bool = False):
-    """Pick (block, n_cg) for ``batch``; returns (block, SolverConfig).
+         verbose: bool = False, cg_forcing: Sequence[float] = CG_FORCING):
+    """Pick (block, n_cg, cg_forcing) for ``batch``; returns (block, SolverConfig).
 
     The trial copies up to ``max_instances`` instances (the longest-running
     ones when lengths differ), runs ``warmup_steps`` untimed steps past the
@@ -72,12 +76,13 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_step
     import torch
     key = _key(batch, mode)
     if key in _cache:
-        blk, ncg = _cache[key]
-        return blk, SolverConfig(mode=mode, n_cg=ncg)
+        blk, ncg, frc = _cache[key]
+        return blk, SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)
     if mode != "refined":
         # reference mode reproduces the reference's algorithm, whose CG count
         # (6, params.py:656) is part of it: only the launch shape is tuned
         n_cg = (6,)
+        cg_forcing = (0.0,)
     W = min(warmup_steps, max(batch.T - 2, 1))
     T = min(W + trial_steps, batch.T)
     order = np.argsort(-batch.lengths, kind="stable")[:max_instances]
@@ -85,32 +90,31 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_step
     dyn = (K, batch.nu[order]) if K is not None else None
     blocks = list(blocks) if blocks is not None else feasible_blocks(batch, mode)
     best = None
-    for blk in blocks:
-        for ncg in n_cg:
-            cfg = SolverConfig(mode=mode, n_cg=ncg)
-            trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
-                          batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn,
-                          pops=batch.pops.subset(order) if batch.pops is not None else None)
-            trial.block = blk
-            trial.run(W, t0=0, cfg=cfg)                      # equilibrate + warm-up
-            s = torch.cuda.current_stream()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(s)
-            trial.run(T - W, t0=W, cfg=cfg, sync=False)
-            e1.record(s)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1)
-            resid = float(trial.metrics()[:, :T, 1].max())
-            ok = mode != "refined" or resid <= cfg.refine_tol * (1 + 1e-6)
-            if verbose:
-                print(f"autotune block {blk} n_cg {ncg}: {ms / (T - W) * 1e3:.1f} us/step, "
-                      f"max residual {resid:.2e}{'' if ok else ' (rejected)'}")
-            if ok and (best is None or ms < best[0]):
-                best = (ms, blk, ncg)
-            del trial
+    for blk, ncg, frc in [(b_, n_, f_) for b_ in blocks for n_ in n_cg for f_ in cg_forcing]:
+        cfg = SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)
+        trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
+                      batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn,
+                      pops=batch.pops.subset(order) if batch.pops is not None else None)
+        trial.block = blk
+        trial.run(W, t0=0, cfg=cfg)                      # equilibrate + warm-up
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        trial.run(T - W, t0=W, cfg=cfg, sync=False)
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        resid = float(trial.metrics()[:, :T, 1].max())
+        ok = mode != "refined" or resid <= cfg.refine_tol * (1 + 1e-6)
+        if verbose:
+            print(f"autotune block {blk} n_cg {ncg} forcing {frc}: {ms / (T - W) * 1e3:.1f} us/step, "
+                  f"max residual {resid:.2e}{'' if ok else ' (rejected)'}")
+        if ok and (best is None or ms < best[0]):
+            best = (ms, blk, ncg, frc)
+        del trial
     if best is None:
-        blk, ncg = default_block(batch.pack.topos), 6
+        blk, ncg, frc = default_block(batch.pack.topos), 6, 0.0
     else:
-        _, blk, ncg = best
-    _cache[key] = (blk, ncg)
-    return blk, SolverConfig(mode=mode, n_cg=ncg)
+        _, blk, ncg, frc = best
+    _cache[key] = (blk, ncg, frc)
+    return blk, SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)

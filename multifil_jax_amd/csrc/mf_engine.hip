@@ -846,8 +846,10 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
 // arithmetic): u = M r and w = A u are formed first and (r.u, w.u) reduced
 // together, so each iteration costs one fused reduction and two barriers;
 // p and s = A p follow from u, w by the beta recurrence.  Refined mode only
-// needs the correction to converge: the FP64 residual decides when to stop,
-// and CG stops early once r.u reaches FP32 rounding.  (The pipelined variant
+// needs the correction to the accuracy the Newton step can use: it stops
+// once r.u / r0.u0 <= eta2 (an inexact-Newton forcing term set by the
+// caller), or as soon as the ratio of the last two r.u predicts that the
+// update just made reaches it, and always once r.u reaches FP32 rounding.  (The pipelined variant
 // that shares one barrier between the reduction and the matvec was tried: its
 // recurred residual drifts from the true one once converged, and passive
 // instances, which converge in one or two passes, then lose accuracy.)
@@ -861,7 +863,8 @@ template <int BS, int MS, bool REF64, bool DYN, bool GW>
 __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                     const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
                                     const float (&cb)[MS], const Radial& rad, float b_d, int n_cg,
-                                    float (&x)[MS], float& x_d, Red<BS>& rd, int lane, int wave) {
+                                    float eta2, float (&x)[MS], float& x_d, Red<BS>& rd, int lane,
+                                    int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
   const float ikk = 1.0f / kk, ikn = 1.0f / kn;
   const bool dyn = DYN && rad.on;
@@ -900,7 +903,9 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       // broken down: further passes only add rounding noise (every lane sees
       // the same reduced g, so the exit is uniform)
       if (it == 0) gamma0 = g;
-      if (!(g > 1e-14f * gamma0)) break;
+      if (!(g > eta2 * gamma0)) break;
+      // r.u shrinks by a near-constant factor per pass: predict the next one
+      const bool last = eta2 > 1e-14f && it > 0 && g * g <= eta2 * gamma0 * gamma;
       const float beta = it == 0 ? 0.0f : g / (gamma + 1e-30f);
       const float den = it == 0 ? dl : dl - beta * g / alpha;
       alpha = g / (den + 1e-30f);
@@ -918,7 +923,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
         x_d += alpha * p_d;
         r_d -= alpha * q_d;
       }
-      if (it >= n_cg - 1) break;
+      if (last || it >= n_cg - 1) break;
     }
     return;
   }
@@ -1014,7 +1019,17 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
 #pragma unroll
     for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
     float dd;
-    pcg<BS, MS, REF64, DYN>(sh, dm, sl, pt, fb, ktip, cb, rad, rad.frad, cfg.n_cg, dx, dd, rd, lane, wave);
+    // forcing term: the step's linear error (max-norm ~ 10 eta |F|) well
+    // below the tolerance, but no tighter than what the nonlinearity leaves
+    // after one Newton step anyway (quadratic: ~1e-5 |F| relative, |F| in
+    // pN); FP32 rounding bounds it from below
+    float eta2 = 1e-14f;
+    if (REF64 && cfg.cg_forcing > 0.0f) {
+      const float eta = fmaxf(cfg.cg_forcing * tol / mx, 1e-5f * mx);
+      eta2 = fmaxf(eta * eta, 1e-14f);
+    }
+    pcg<BS, MS, REF64, DYN>(sh, dm, sl, pt, fb, ktip, cb, rad, rad.frad, cfg.n_cg, eta2, dx, dd, rd, lane,
+                            wave);
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
       const int n = sl.node[s];

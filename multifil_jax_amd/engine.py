@@ -63,6 +63,7 @@ class SolverConfig:
     tol: Optional[float] = None
     refine_tol: float = 1e-5
     dt: float = 1.0
+    cg_forcing: float = 0.0
 
     def ctype(self) -> _lib.MfSolverCfg:
         if self.mode not in ("reference", "refined"):
@@ -71,7 +72,7 @@ class SolverConfig:
             mode=_lib.MF_SOLVER_REFINED if self.mode == "refined" else _lib.MF_SOLVER_REFERENCE,
             n_newton=self.n_newton, n_cg=self.n_cg, n_refine=self.n_refine,
             tol=float("nan") if self.tol is None else float(self.tol),
-            refine_tol=float(self.refine_tol), dt=float(self.dt))
+            refine_tol=float(self.refine_tol), dt=float(self.dt), cg_forcing=float(self.cg_forcing))
 
 
 class LatticePack:

@@ -61,3 +61,35 @@ def test_autotune_picks_a_converging_config(torch_cuda, topo22, skeletal):
     bt.block = blk
     bt.run(T, cfg=cfg)
     assert np.all(bt.metrics()[:, :, 1] <= 1e-5)
+
+
+def test_cg_forcing_keeps_the_refined_tolerance(torch_cuda, topo22, skeletal):
+    """The inexact-Newton CG stop changes how the solve gets there, not where.
+    Free runs diverge (a 1e-8 nm difference flips a float32 nearest-site
+    comparison now and then), so the check is teacher-forced: the same
+    state, one step, fixed-count CG against the forcing term. The kinetics
+    agree exactly; both solves meet 1e-5 pN, and their M-line forces agree
+    to far below the force scale; over a free run every step meets the
+    tolerance."""
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    _, d = skeletal
+    B, T = 16, 30
+    drv = np.zeros((B, T, 3), np.float32)
+    drv[..., 0], drv[..., 1], drv[..., 2] = np.linspace(850, 1150, B)[:, None], np.linspace(4.0, 7.0, B)[:, None], 14.0
+    pack = LatticePack([topo22])
+    mk = lambda: Batch(pack, np.zeros(B, np.int32), d.rows(B), drv, seeds=np.arange(B, dtype=np.uint64))
+    free = mk()
+    free.run(T, cfg=SolverConfig(mode="refined", cg_forcing=0.003))
+    assert np.all(free.metrics()[:, :, 1] <= 1e-5)
+    states = [free.get_state(b) for b in range(B)]
+    out = []
+    for forcing in (0.0, 0.003):
+        bt = mk()
+        for b in range(B):
+            bt.set_state(b, states[b])
+        bt.run(1, t0=T - 1, cfg=SolverConfig(mode="refined", cg_forcing=forcing))
+        out.append((bt.force()[:, T - 1], bt.metrics()[:, T - 1]))
+    (f0, m0), (f1, m1) = out
+    assert np.array_equal(m0[:, 5:12], m1[:, 5:12]) and np.array_equal(m0[:, 19:23], m1[:, 19:23])
+    assert np.all(m0[:, 1] <= 1e-5) and np.all(m1[:, 1] <= 1e-5)
+    assert np.max(np.abs(f0 - f1)) < 1e-3
