@@ -15,7 +15,9 @@ Also reported:
 * the plain per-condition GPU form (B = 1, one launch per timestep) on a
   bounded sample;
 * the largest solver residual of the timed run;
-* the CPU baseline: the FP64 oracle on a bounded sample.
+* the CPU baseline: the C restatement of the step (oracle/cpu_port.c, FP32,
+  the reference solver's caps) over the same sweep on up to 16 host
+  threads.
 
 The force-error bar of the metric (<= 1e-5 relative vs FP64) is checked
 against the oracle by the GPU tests, not here.
