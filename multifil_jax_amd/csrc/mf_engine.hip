@@ -1675,7 +1675,10 @@ const Variant* variants(int& n) {
       make_variant<512, 8, 8, false>(),  make_variant<512, 8, 8, true>(),
       make_variant<256, 8, 4, false>(),  make_variant<256, 8, 4, true>(),
       make_variant<256, 16, 16, false>(), make_variant<256, 16, 16, true>(),
-      // global-workspace variants, for lattices too large for LDS (6x6)
+      // global-workspace variants, for lattices too large for LDS (6x6: 12
+      // node slots and 6 heads per thread; larger lattices take 16 / 8)
+      make_variant<1024, 12, 6, false, false, true>(), make_variant<1024, 12, 6, true, false, true>(),
+      make_variant<1024, 12, 6, false, true, true>(),  make_variant<1024, 12, 6, true, true, true>(),
       make_variant<1024, 16, 8, false, false, true>(), make_variant<1024, 16, 8, true, false, true>(),
       make_variant<1024, 16, 8, false, true, true>(),  make_variant<1024, 16, 8, true, true, true>(),
   };
