@@ -1478,8 +1478,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       const LDS unsigned char* tmc_p = sh.tm + tmc * S;
       const double kk = (double)sh.prm[P_THICK_K];
       const double ta = sh.prm[P_TITIN_A], tbp = sh.prm[P_TITIN_B], trest = sh.prm[P_TITIN_REST];
-      // pass 1: node sums and extrema (wave partials), per-filament values
+      // one pass: node sums, sums of squares and extrema (wave partials),
+      // per-filament values
       double sk = 0, sn = 0, wk = 0, mxk = -INFINITY, nmk = -INFINITY, mxn = -INFINITY, nmn = -INFINITY;
+      double qk = 0, qn = 0;
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
         const int n = sl.node[s];
@@ -1488,7 +1490,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const double o = (double)sh.org[n];
         const double dd = x - o;
         if (sl.thick[s]) {
-          sk += dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
+          sk += dd; qk += dd * dd; mxk = fmax(mxk, dd); nmk = fmax(nmk, -dd);
           wk += x;
           LDS double* fv = sh.fil + sl.fil[s] * NF;
           if (sl.li[s] == 0) {
@@ -1507,16 +1509,18 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             fv[F_DTIT] = en - eo;
           }
         } else {
-          sn += dd; mxn = fmax(mxn, dd); nmn = fmax(nmn, -dd);
+          sn += dd; qn += dd * dd; mxn = fmax(mxn, dd); nmn = fmax(nmn, -dd);
         }
       }
       sk = wave_sum(sk, lane); sn = wave_sum(sn, lane); wk = wave_sum(wk, lane);
+      qk = wave_sum(qk, lane); qn = wave_sum(qn, lane);
       mxk = wave_max(mxk, lane); nmk = wave_max(nmk, lane);
       mxn = wave_max(mxn, lane); nmn = wave_max(nmn, lane);
       if (lane == 0) {
         LDS double* wp = sh.wpart + wave * NWP;
         wp[W_SUMK] = sk; wp[W_SUMN] = sn; wp[W_WORK] = wk;
         wp[W_MAXK] = mxk; wp[W_NMINK] = nmk; wp[W_MAXN] = mxn; wp[W_NMINN] = nmn;
+        wp[W_SQK] = qk; wp[W_SQN] = qn;
       }
       // TM state and overlap counts on the post-solve positions
       for (int s2 = tid; s2 < S + (BS - S % BS) % BS; s2 += BS) {
@@ -1533,24 +1537,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         wave_count(&sh.cnt[C_OV2], ov && st == 2, lane);
         wave_count(&sh.cnt[C_OV3], ov && st == 3, lane);
       }
-      __syncthreads();
-      double sumk = 0, sumn = 0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) { sumk += sh.wpart[w * NWP + W_SUMK]; sumn += sh.wpart[w * NWP + W_SUMN]; }
-      const double mean_k = sumk * inv_nk, mean_n = sumn * inv_S;
-      // pass 2: population variances (jnp.std)
-      double qk = 0, qn = 0;
-#pragma unroll
-      for (int s = 0; s < MS; ++s) {
-        const int n = sl.node[s];
-        if (n < 0) continue;
-        const double dd = posd<REF64>(sh, n) - (double)sh.org[n];
-        if (sl.thick[s]) qk += (dd - mean_k) * (dd - mean_k);
-        else qn += (dd - mean_n) * (dd - mean_n);
-      }
-      qk = wave_sum(qk, lane);
-      qn = wave_sum(qn, lane);
-      if (lane == 0) { sh.wpart[wave * NWP + W_SQK] = qk; sh.wpart[wave * NWP + W_SQN] = qn; }
       __syncthreads();
       if (wave == 0) {
         // lane k < NWP folds wave-partial column k, lanes 16.. fold the
@@ -1571,6 +1557,12 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           sh.cnt[lane - 32] = 0;
         }
         const double force = read_lane(v, 16 + F_FORCE);
+        // population variances (jnp.std) from the sums of squares, in double
+        // (displacements are O(100) nm at most: the cancellation costs a few
+        // of double's 16 digits, far from the float32 result)
+        const double mean_k = read_lane(v, W_SUMK) * inv_nk, mean_n = read_lane(v, W_SUMN) * inv_S;
+        const double var_k = fmax(read_lane(v, W_SQK) * inv_nk - mean_k * mean_k, 0.0);
+        const double var_n = fmax(read_lane(v, W_SQN) * inv_S - mean_n * mean_n, 0.0);
         const double work = force * ((read_lane(v, W_WORK) - read_lane(v, W_XPRE)) * inv_nk);
         const double atpe = read_lane(v, W_ATPE);
         if (lane == 0) {
@@ -1600,9 +1592,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             m[31] = ov;
             m[32] = c[C_ATPC]; m[33] = c[C_NEWLY];
             m[34] = (float)mean_k; m[35] = (float)read_lane(v, W_MAXK); m[36] = (float)(-read_lane(v, W_NMINK));
-            m[37] = (float)sqrt(read_lane(v, W_SQK) * inv_nk);
+            m[37] = (float)sqrt(var_k);
             m[38] = (float)mean_n; m[39] = (float)read_lane(v, W_MAXN); m[40] = (float)(-read_lane(v, W_NMINN));
-            m[41] = (float)sqrt(read_lane(v, W_SQN) * inv_S);
+            m[41] = (float)sqrt(var_n);
             m[42] = (float)read_lane(v, 16 + F_E1) * invTk; m[43] = (float)read_lane(v, 16 + F_DE1) * invTk;
             m[44] = (float)read_lane(v, 16 + F_TIT) * invTk; m[45] = (float)read_lane(v, 16 + F_DTIT) * invTk;
             m[46] = (float)work; m[47] = (float)(work * inv_nk);
