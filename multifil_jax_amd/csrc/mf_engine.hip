@@ -849,10 +849,11 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
 // needs the correction to the accuracy the Newton step can use: it stops
 // once r.u / r0.u0 <= eta2 (an inexact-Newton forcing term set by the
 // caller), or as soon as the ratio of the last two r.u predicts that the
-// update just made reaches it, and always once r.u reaches FP32 rounding.  (The pipelined variant
-// that shares one barrier between the reduction and the matvec was tried: its
-// recurred residual drifts from the true one once converged, and passive
-// instances, which converge in one or two passes, then lose accuracy.)
+// update just made reaches it, and always once r.u reaches FP32 rounding.
+// (The pipelined variant that shares one barrier between the reduction and
+// the matvec was tried: its recurred residual drifts from the true one once
+// converged, and passive instances, which converge in one or two passes,
+// then lose accuracy.)
 //
 // With dynamic spacing the system is bordered: the unknown d (uniform, held
 // by every lane) couples to the nodes through the column cb, so
