@@ -304,6 +304,7 @@ struct Shared {
   LDS float* red;        // [8][NW] scalar reduction ring
   LDS double* wpart;     // [NW][NWP] metric wave partials
   LDS double* fil;       // [Tk][NF] metric per-filament values
+  LDS float* ktip;       // [Tk] titin tip stiffness of each thick filament (last residual)
   LDS int* cnt;          // [NC] metric counters
 };
 
@@ -313,7 +314,7 @@ struct Shared {
 // the workspace slice when GW is set.
 struct LdsOffsets {
   int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
-      edges, prm, red, wpart, fil, cnt, total, ws_total;
+      edges, prm, red, wpart, fil, ktip, cnt, total, ws_total;
 };
 template <int BS, bool REF64, bool GW = false>
 __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
@@ -350,6 +351,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.red = take(4 * 8 * NW);
   L.wpart = take(8LL * NW * NWP);
   L.fil = take(8LL * d.Tk * NF);
+  L.ktip = take(4LL * d.Tk);
   L.cnt = take(4 * NC);
   L.total = o;
   L.ws_total = w;
@@ -394,6 +396,7 @@ __device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, con
   t.red = (LDS float*)(base + o.red);
   t.wpart = (LDS double*)(base + o.wpart);
   t.fil = (LDS double*)(base + o.fil);
+  t.ktip = (LDS float*)(base + o.ktip);
   t.cnt = (LDS int*)(base + o.cnt);
   return t;
 }
@@ -656,14 +659,15 @@ struct Radial {
 // Residual at the current positions: per-XB forces/stiffnesses (XB phase),
 // then per-node passive + XB sums (node phase), which also refreshes the
 // partners' stiffnesses.  Returns max|F| over the workgroup; f[] receives the
-// lane's slot residuals, ktip the tip stiffness.  With dynamic spacing the
-// XB phase also forms d f / d d per head (kept in the CG-direction buffer,
-// idle here) and the radial sums; the node phase gathers the border column
-// cb = d F_axial / d d = d F_rad / d x, and the radial residual joins the max.
+// lane's slot residuals, sh.ktip each thick filament's titin tip stiffness.
+// With dynamic spacing the XB phase also forms d f / d d per head (kept in
+// the CG-direction buffer, idle here) and the radial sums; the node phase
+// gathers the border column cb = d F_axial / d d = d F_rad / d x, and the
+// radial residual joins the max.
 template <int BS, int MS, bool REF64, bool DYN, typename T, bool GW>
 __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
-                                          T (&f)[MS], float (&ktip)[MS], float (&cb)[MS], Partners<MS>& pt,
+                                          T (&f)[MS], float (&cb)[MS], Partners<MS>& pt,
                                           Radial& rad, Red<BS>& rd, int tid, int lane, int wave) {
   const LDS float* prm = sh.prm;
   const bool dyn = DYN && rad.on;
@@ -698,7 +702,6 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     f[s] = T(0);
-    ktip[s] = 0.0f;
     cb[s] = 0.0f;
     const int n = sl.node[s];
     if (n < 0) continue;
@@ -732,7 +735,9 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       if (i < dm.nc - 1) {
         s_hi = (T(posd<REF64>(sh, n + 1)) - x0 - T(rB[s])) * kk;
       } else {
-        titin<T>(prm, x0, zl, d, s_hi, ktip[s]);
+        float kt;
+        titin<T>(prm, x0, zl, d, s_hi, kt);
+        sh.ktip[sl.fil[s]] = kt;
         if (dyn) {
           T tr;
           float trd, ttd;
@@ -790,7 +795,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
 // wave-uniform; only the rare over-full site list takes a branch).
 template <int MS, bool REF64, bool GW, typename V>
 __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
-                                        const Partners<MS>& pt, const float (&ktip)[MS], float kk, float kn,
+                                        const Partners<MS>& pt, float kk, float kn,
                                         V pv, float (&Ap)[MS]) {
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
@@ -820,7 +825,7 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
       const bool last = i >= dm.nc - 1;
       const float pnext = pv[last ? n : n + 1];
       const float lo = kk * (p0 - (i > 0 ? pprev : 0.0f));
-      const float hi = last ? ktip[s] * p0 : kk * (pnext - p0);
+      const float hi = last ? sh.ktip[sl.fil[s]] * p0 : kk * (pnext - p0);
       net = hi - lo + q;
     } else {
       const bool last = i >= dm.ns - 1;
@@ -862,8 +867,7 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
 // iteration's existing reduction.
 template <int BS, int MS, bool REF64, bool DYN, bool GW>
 __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
-                                    const Partners<MS>& pt, const float (&b)[MS], const float (&ktip)[MS],
-                                    const float (&cb)[MS], const Radial& rad, float b_d, int n_cg,
+                                    const Partners<MS>& pt, const float (&b)[MS], const float (&cb)[MS], const Radial& rad, float b_d, int n_cg,
                                     float eta2, float (&x)[MS], float& x_d, Red<BS>& rd, int lane,
                                     int wave) {
   const float kk = sh.prm[P_THICK_K], kn = sh.prm[P_THIN_K];
@@ -876,14 +880,25 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     for (int s = 0; s < MS; ++s) { r[s] = b[s]; x[s] = 0.0f; p[s] = 0.0f; q[s] = 0.0f; }
     float r_d = dyn ? b_d : 0.0f, p_d = 0.0f, q_d = 0.0f;
     float gamma = 0.0f, alpha = 1.0f, gamma0 = 0.0f;
+    Slots<MS> sl_ = sl;
     for (int it = 0;; ++it) {
+      if constexpr (GW) {
+        // Hide the node indices from loop-invariant hoisting: with 12 node
+        // slots per lane the compiler would otherwise keep per-slot
+        // addresses, masks and scale factors live across iterations and
+        // spill them, reloading each one every pass; recomputing them from
+        // the index is a few VALU operations.
+#pragma unroll
+        for (int s = 0; s < MS; ++s) asm volatile("" : "+v"(sl_.node[s]), "+v"(sl_.li[s]));
+      }
+      const Slots<MS>& sl = sl_;
       precond<MS>(r, u, sl, lane, ikk, ikn);
       const float u_d = rad.dinv * r_d;
 #pragma unroll
       for (int s = 0; s < MS; ++s)
         if (sl.node[s] >= 0) sh.pv[sl.node[s]] = u[s];
       __syncthreads();
-      neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, sh.pv, w);
+      neg_jvp<MS, REF64>(sh, dm, sl, pt, kk, kn, sh.pv, w);
       float g = 0.0f, dl = 0.0f, cu = 0.0f;
       if (dyn) {
 #pragma unroll
@@ -950,7 +965,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     }
     p_d = it == 0 ? z_d : z_d + beta * p_d;
     __syncthreads();
-    neg_jvp<MS, REF64>(sh, dm, sl, pt, ktip, kk, kn, sh.pv, Ap);
+    neg_jvp<MS, REF64>(sh, dm, sl, pt, kk, kn, sh.pv, Ap);
     loc = 0.0f;
     float pAp, Ap_d = 0.0f;
     if (dyn) {
@@ -994,7 +1009,7 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
                                       int wave, int& iters, float& resid) {
   using T = typename std::conditional<REF64, double, float>::type;
   T f[MS];
-  float ktip[MS], dx[MS], fb[MS], cb[MS];
+  float dx[MS], fb[MS], cb[MS];
   float tol;
   int cap;
   if constexpr (REF64) {
@@ -1010,7 +1025,7 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
   int it = 0;
   float mx, prev = INFINITY;
   for (;;) {
-    mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, f, ktip, cb, pt, rad, rd, tid, lane,
+    mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, f, cb, pt, rad, rd, tid, lane,
                                          wave);
     // the d-block preconditioner uses J_dd at the initial state (solver.py:633-638)
     if (DYN && rad.on && it == 0) rad.dinv = -1.0f / rad.jdd;
@@ -1029,7 +1044,7 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
       const float eta = fmaxf(cfg.cg_forcing * tol / mx, 1e-5f * mx);
       eta2 = fmaxf(eta * eta, 1e-14f);
     }
-    pcg<BS, MS, REF64, DYN>(sh, dm, sl, pt, fb, ktip, cb, rad, rad.frad, cfg.n_cg, eta2, dx, dd, rd, lane,
+    pcg<BS, MS, REF64, DYN>(sh, dm, sl, pt, fb, cb, rad, rad.frad, cfg.n_cg, eta2, dx, dd, rd, lane,
                             wave);
 #pragma unroll
     for (int s = 0; s < MS; ++s) {
