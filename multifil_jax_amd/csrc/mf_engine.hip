@@ -290,6 +290,8 @@ struct Shared {
   Big<short> xkey;       // [X]   table row of the last kinetics step (< 2 * MF_MAX_BINS)
   Big<short> xslot;      // [X]   (thin, face) slot | target-valid << 15
   Big<float> org;        // [N]   displacement origin: crown offset / cumulative thin rest
+  Big<float> cbn;        // [N]   GW + dynamic spacing: the border column per node, which
+                         //       other variants keep in registers
   LDS unsigned char* xst;  // [X] XB state
   LDS int* shead;        // [S]   first XB bound to the site, -1 none
   LDS int* tbound;       // [S]   thin.bound_to
@@ -313,7 +315,7 @@ struct Shared {
 // pointers), so the two cannot disagree.  Offsets of "Big" arrays are into
 // the workspace slice when GW is set.
 struct LdsOffsets {
-  int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
+  int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, cbn, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
       edges, prm, red, wpart, fil, ktip, cnt, total, ws_total;
 };
 template <int BS, bool REF64, bool GW = false>
@@ -337,6 +339,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.xkey = big(2LL * d.X);
   L.xslot = big(2LL * d.X);
   L.org = big(4LL * d.N);
+  L.cbn = GW ? big(4LL * d.N) : 0;
   L.xst = take(d.X);
   L.shead = take(4LL * d.S);
   L.tbound = take(4LL * d.S);
@@ -382,6 +385,7 @@ __device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, con
   t.xkey = (typename S::template Big<short>)bb(o.xkey);
   t.xslot = (typename S::template Big<short>)bb(o.xslot);
   t.org = (typename S::template Big<float>)bb(o.org);
+  t.cbn = (typename S::template Big<float>)bb(o.cbn);
   t.xst = (LDS unsigned char*)(base + o.xst);
   t.shead = (LDS int*)(base + o.shead);
   t.tbound = (LDS int*)(base + o.tbound);
@@ -750,6 +754,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       net = s_hi - s_lo;
       net += xs;
       cb[s] = cs;
+      if (GW && dyn) sh.cbn[n] = cs;
     } else {
       const T xp = i < dm.ns - 1 ? T(posd<REF64>(sh, n + 1)) : zl;
       const T s_hi = (xp - x0 - T(rA[s])) * kn;
@@ -765,6 +770,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       }
       net -= xs;
       cb[s] = -cs;
+      if (GW && dyn) sh.cbn[n] = -cs;
     }
     f[s] = net;
     // max|F| is taken in float: it is compared with the stopping tolerance
@@ -902,7 +908,11 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       float g = 0.0f, dl = 0.0f, cu = 0.0f;
       if (dyn) {
 #pragma unroll
-        for (int s = 0; s < MS; ++s) { w[s] -= cb[s] * u_d; cu += cb[s] * u[s]; }
+        for (int s = 0; s < MS; ++s) {
+          const float c = GW ? (sl.node[s] >= 0 ? sh.cbn[sl.node[s]] : 0.0f) : cb[s];
+          w[s] -= c * u_d;
+          cu += c * u[s];
+        }
       }
 #pragma unroll
       for (int s = 0; s < MS; ++s) { g += r[s] * u[s]; dl += w[s] * u[s]; }
@@ -971,7 +981,12 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     if (dyn) {
       float cp = 0.0f;
 #pragma unroll
-      for (int s = 0; s < MS; ++s) { Ap[s] -= cb[s] * p_d; cp += cb[s] * p[s]; loc += p[s] * Ap[s]; }
+      for (int s = 0; s < MS; ++s) {
+        const float c = GW ? (sl.node[s] >= 0 ? sh.cbn[sl.node[s]] : 0.0f) : cb[s];
+        Ap[s] -= c * p_d;
+        cp += c * p[s];
+        loc += p[s] * Ap[s];
+      }
       wg_sum2<BS>(rd, loc, cp, lane, wave);
       Ap_d = -cp - rad.jdd * p_d;
       pAp = loc + p_d * Ap_d;
