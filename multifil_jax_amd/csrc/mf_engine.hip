@@ -671,7 +671,7 @@ struct Radial {
 template <int BS, int MS, bool REF64, bool DYN, typename T, bool GW>
 __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
-                                          T (&f)[MS], float (&cb)[MS], Partners<MS>& pt,
+                                          float (&f)[MS], float (&cb)[MS], Partners<MS>& pt,
                                           Radial& rad, Red<BS>& rd, int tid, int lane, int wave) {
   const LDS float* prm = sh.prm;
   const bool dyn = DYN && rad.on;
@@ -705,7 +705,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
   float mx = 0.0f;
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
-    f[s] = T(0);
+    f[s] = 0.0f;
     cb[s] = 0.0f;
     const int n = sl.node[s];
     if (n < 0) continue;
@@ -772,7 +772,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       cb[s] = -cs;
       if (GW && dyn) sh.cbn[n] = -cs;
     }
-    f[s] = net;
+    f[s] = (float)net;  // the CG right-hand side is FP32
     // max|F| is taken in float: it is compared with the stopping tolerance
     // and reported as a float32 metric.
     mx = fmaxf(mx, fabsf((float)net));
@@ -888,8 +888,8 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     float gamma = 0.0f, alpha = 1.0f, gamma0 = 0.0f;
     Slots<MS> sl_ = sl;
     for (int it = 0;; ++it) {
-      if constexpr (GW) {
-        // Hide the node indices from loop-invariant hoisting: with 12 node
+      if constexpr (GW || MS >= 5) {
+        // Hide the node indices from loop-invariant hoisting: with 5-12 node
         // slots per lane the compiler would otherwise keep per-slot
         // addresses, masks and scale factors live across iterations and
         // spill them, reloading each one every pass; recomputing them from
@@ -1023,7 +1023,6 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
                                       const mf_solver_cfg& cfg, Radial& rad, Red<BS>& rd, int tid, int lane,
                                       int wave, int& iters, float& resid) {
   using T = typename std::conditional<REF64, double, float>::type;
-  T f[MS];
   float dx[MS], fb[MS], cb[MS];
   float tol;
   int cap;
@@ -1040,15 +1039,13 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
   int it = 0;
   float mx, prev = INFINITY;
   for (;;) {
-    mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, f, cb, pt, rad, rd, tid, lane,
+    mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, fb, cb, pt, rad, rd, tid, lane,
                                          wave);
     // the d-block preconditioner uses J_dd at the initial state (solver.py:633-638)
     if (DYN && rad.on && it == 0) rad.dinv = -1.0f / rad.jdd;
     if (!(mx > tol)) break;
     if (it >= cap && !(REF64 && it < 4 * cap && mx < 0.5f * prev)) break;
     prev = mx;
-#pragma unroll
-    for (int s = 0; s < MS; ++s) fb[s] = (float)f[s];
     float dd;
     // forcing term: the step's linear error (max-norm ~ 10 eta |F|) well
     // below the tolerance, but no tighter than what the nonlinearity leaves
