@@ -11,7 +11,8 @@ All inputs are resident in HBM before timing.  One bench "step" advances
 every instance one timestep, so value = B * K / elapsed.
 
 Also reported:
-* the roofline record of the persistent step kernel;
+* the roofline record of the persistent step kernel (HBM), and its VALU
+  issue rate against the VALU peak (the bound that applies);
 * the plain per-condition GPU form (B = 1, one launch per timestep) on a
   bounded sample;
 * the largest solver residual of the timed run;
@@ -56,6 +57,11 @@ BYTES_PER_SARC_STEP_2X2 = 21084
 # sarcomere-step.  The profile that holds it records its own command.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "run_kernel_2x2_sweep_traffic.md")
 COPY_PROBE_MIB = 256
+# Instruction mix of the same kernel (rocprofv3 SQ counters), and the VALU
+# issue peak it is read against: one wave64 VALU instruction per CU per
+# cycle (4 SIMD16 units x 4 cycles each), 256 CUs at 2.4 GHz.
+SQ_PROFILE = os.path.join(ROOT, "profiles", "run_kernel_2x2_sweep_sq.md")
+VALU_PEAK_GINST = 256 * 2.4
 
 
 def measured_traffic_per_sarc_step():
@@ -67,6 +73,45 @@ def measured_traffic_per_sarc_step():
                 if line.startswith("traffic_bytes_per_sarcomere_step:"):
                     return float(line.split(":", 1)[1])
     except OSError:
+        pass
+    return None
+
+
+def valu_record(bt, K, ms_kernel):
+    """The kernel against the compute side: VALU instructions issued per
+    second (profiled count per wave-step x waves x steps / live kernel time)
+    over the VALU issue peak.  The step is latency- and VALU-bound, not
+    HBM-bound, so this is the fraction that says how close it runs."""
+    per = measured_valu_per_wave_step()
+    waves = bt.B * (bt.block // 64)
+    if per is None or profiled_waves() != waves:
+        return None             # the profile is of another launch shape
+    achieved = per * waves * K / (ms_kernel / 1e3) / 1e9
+    return {"insts_per_wave_step": per, "waves": waves, "achieved": achieved,
+            "peak": VALU_PEAK_GINST, "unit": "G wave-instr/s", "frac": achieved / VALU_PEAK_GINST}
+
+
+def profiled_waves():
+    """Waves of the profiled dispatch ('Timed dispatch (K steps, W waves = ...')
+    from the SQ profile."""
+    import re
+    try:
+        with open(SQ_PROFILE) as f:
+            m = re.search(r"Timed dispatch \(\d+ steps, (\d+) waves", f.read())
+            return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def measured_valu_per_wave_step():
+    """SQ_INSTS_VALU per wave per step from the committed SQ profile; None
+    when it is absent."""
+    try:
+        with open(SQ_PROFILE) as f:
+            for line in f:
+                if line.startswith("| SQ_INSTS_VALU |"):
+                    return float(line.strip().strip("|").split("|")[-1])
+    except (OSError, ValueError):
         pass
     return None
 
@@ -392,6 +437,7 @@ def main():
                      "kernel": "run_kernel", "kernel_ms": ms_kernel, "launches": 1},
         "host_s": t_host,
         "max_solver_residual_pN": float(bt.metrics()[:, W:W + K, 1].max()),
+        "valu": valu_record(bt, K, ms_kernel),
     }
     if args.copy_probe:
         out["copy_probe"] = copy_probe()
