@@ -94,11 +94,11 @@ def _teacher_forced(torch, topo, d, p, mode, n_starts, pCa, z, seed, steps_betwe
 
 
 @pytest.mark.parametrize("mode", ["refined", "reference"])
-@pytest.mark.parametrize("lat", [2, 4])
+@pytest.mark.parametrize("lat", [2, 3, 4])
 def test_teacher_forced_step(torch_cuda, skeletal, pdict, mode, lat):
     from multifil_jax_amd.topology import SarcTopology
     s, d = skeletal
-    topo = SarcTopology.create(lat, lat, s, d)
+    topo = SarcTopology.create(lat, lat, s, d)   # 3x3: odd rows, 5 node slots per lane
     n_starts = 6 if lat == 2 else 3
     expect, got, met, frc = _teacher_forced(torch_cuda, topo, d, pdict, mode, n_starts,
                                             pCa=4.5, z=1000.0, seed=11)

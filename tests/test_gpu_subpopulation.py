@@ -99,3 +99,38 @@ def test_subpopulation_parity(torch_cuda, topo22, skeletal, pdict, mode):
     assert out[-1, 5] > 0
     assert np.max(np.abs(fg - out[:, 0])[same] / np.abs(out[same, 0])) < 1e-5
     assert np.all(mg[:, 1] <= 1e-5)
+
+
+def test_explicit_subpopulation_with_dynamic_spacing(torch_cuda, topo22, skeletal, pdict):
+    """c_zone labels (XB and TM) on a dynamic-spacing run: the per-label
+    tables are rebuilt in the kernel at the carried and at the solved
+    spacing each step; parity against the oracle as above, plus the solved
+    spacing within 1e-6 nm."""
+    from multifil_jax_amd import Subpopulation, run
+    from multifil_jax_amd.simulation import plan_sweep
+    from multifil_jax_amd.subpopulation import population_params
+    from multifil_jax_amd.params import DYNAMIC_FIELDS
+    s, d = skeletal
+    sp = Subpopulation.c_zone(topo22, **SCALES)
+    T, K = 12, 5.0
+    res = run(topo22, duration_ms=T, pCa=4.5, z_line=950.0, rng_seed=3, solver="refined",
+              static_params=s, dynamic_params=d, subpopulation=sp, K_lat=K)
+    plan = plan_sweep(topo22, duration_ms=T, pCa=4.5, z_line=950.0, dynamic_params=d, subpopulation=sp,
+                      K_lat=K)
+    rp = plan.subpop
+    P = population_params(plan.params, rp)[0]
+    pops = [{n: float(P[k, i]) for i, n in enumerate(DYNAMIC_FIELDS)} for k in range(rp.K)]
+    pop = mo.Pop(rp.mode, pops, rp.fractions[0], has_xb=rp.has_xb, has_tm=rp.has_tm,
+                 xb_label=rp.xb_mask[0], tm_label=rp.tm_mask[0])
+    drv = plan.drivers[0]
+    _, out = mo.run(topo22, pdict, drv[:, 0], drv[:, 1], drv[:, 2], 1.0, (3 << 32) | 0,
+                    mo.SolverConfig(mode="exact"), K_lat=float(plan.K_lat[0]), nu=0.0, pop=pop)
+    mg = np.stack([res.metrics[k].reshape(-1) for k in mo.METRIC_NAMES], axis=1)
+    fg = res.axial_force.reshape(-1)
+    same = np.all(mg[:, 6:12] == out[:, 6:12], axis=1) & np.all(mg[:, 19:23] == out[:, 19:23], axis=1)
+    assert same[:6].all()
+    assert np.max(np.abs(fg - out[:, 0])[same] / np.abs(out[same, 0])) < 1e-5
+    assert np.max(np.abs(mg[same, 4] - out[same, 4])) < 1e-6 * 14 + 2e-6
+    for col in (48, 49):     # expected ATP / tear from the per-label absorbing rows
+        assert np.all(np.abs(mg[same, col] - out[same, col]) <= 2e-5 * np.abs(out[same, col]) + 1e-5), col
+    assert np.all(mg[:, 1] <= 1e-5)
