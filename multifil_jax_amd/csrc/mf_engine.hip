@@ -552,11 +552,22 @@ template <typename T>
 __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T th, T& f, float& kf) {
   const T r = sqrt(xd * xd + d * d);
   const T rs = r > T(1e-10) ? r : T(1e-10);
-  const T cs = xd / rs, sn = d / rs;
-  f = T(sp.gk) * (r - T(sp.gr)) * cs - (T(1) / rs) * T(sp.ck) * (th - T(sp.cr)) * sn;
-  const T r2 = rs * rs;
-  kf = (float)(T(sp.gk) * (T(1) - T(sp.gr) * d * d / (r2 * rs)) +
-               T(sp.ck) * d * (d + T(2) * xd * (th - T(sp.cr))) / (r2 * r2));
+  if constexpr (std::is_same<T, double>::value) {
+    // refined mode: one double division for the force; the stiffness only
+    // steers CG, so it is formed in float from the float reciprocal
+    const double inv = 1.0 / rs;
+    f = (double(sp.gk) * (r - double(sp.gr)) * xd - inv * double(sp.ck) * (th - double(sp.cr)) * d) * inv;
+    const float dfl = (float)d, xf = (float)xd, dth = (float)th - sp.cr;
+    const float i1 = __builtin_amdgcn_rcpf((float)rs), i2 = i1 * i1;
+    kf = sp.gk * (1.0f - sp.gr * dfl * dfl * (i2 * i1)) + sp.ck * dfl * (dfl + 2.0f * xf * dth) * (i2 * i2);
+  } else {
+    // reference mode: the reference's float expression (forces.py:398-399)
+    const T cs = xd / rs, sn = d / rs;
+    f = T(sp.gk) * (r - T(sp.gr)) * cs - (T(1) / rs) * T(sp.ck) * (th - T(sp.cr)) * sn;
+    const T r2 = rs * rs;
+    kf = (float)(T(sp.gk) * (T(1) - T(sp.gr) * d * d / (r2 * rs)) +
+                 T(sp.ck) * d * (d + T(2) * xd * (th - T(sp.cr))) / (r2 * r2));
+  }
 }
 
 // Spacing derivatives of a bound XB, for the bordered system of dynamic
