@@ -255,7 +255,7 @@ struct Partners {
 __device__ __forceinline__ int p_node(int pk) { return pk >> 16; }
 __device__ __forceinline__ int p_xb(int pk) { return pk & 0xffff; }
 
-// Integer metric counters (LDS, reset by thread 0 after each step's output).
+// Integer metric counters (LDS, one row per wave, rewritten every step).
 enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, NC = 16 };
 // Per-thick-filament metric values written by the owning lane; the first
 // NF_SUM are summed over filaments, F_X0 / F_XT hold the pre-solve positions
@@ -307,7 +307,7 @@ struct Shared {
   LDS double* wpart;     // [NW][NWP] metric wave partials
   LDS double* fil;       // [Tk][NF] metric per-filament values
   LDS float* ktip;       // [Tk] titin tip stiffness of each thick filament (last residual)
-  LDS int* cnt;          // [NC] metric counters
+  LDS int* cnt;          // [NW][NC] metric counters, one row per wave
 };
 
 // LDS / workspace carve-up.  One routine computes the byte offsets (host:
@@ -355,7 +355,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.wpart = take(8LL * NW * NWP);
   L.fil = take(8LL * d.Tk * NF);
   L.ktip = take(4LL * d.Tk);
-  L.cnt = take(4 * NC);
+  L.cnt = take(4 * NC * NW);
   L.total = o;
   L.ws_total = w;
   return L;
@@ -1133,10 +1133,8 @@ __device__ __forceinline__ int bin_of(const LDS float* edges, int nb, float inv_
 }
 
 // Add popcount(pred over the wave) to an LDS counter (one atomic per wave).
-__device__ __forceinline__ void wave_count(LDS int* c, bool pred, int lane) {
-  const unsigned long long m = __ballot(pred);
-  if (lane == 0 && m) atomicAdd((int*)c, (int)__popcll(m));
-}
+// Lanes of the wave for which pred holds (wave-uniform result).
+__device__ __forceinline__ int wave_count(bool pred) { return __popcll(__ballot(pred)); }
 
 template <int BS, int MS, int MX, bool REF64, bool DYN, bool GW>
 __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
@@ -1184,7 +1182,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     sh.tprev[i] = (signed char)(li[G.tm_prev + i] - k);
     sh.tnext[i] = (signed char)(li[G.tm_next + i] - k);
   }
-  for (int i = tid; i < NC; i += BS) sh.cnt[i] = 0;
+  for (int i = tid; i < NC * NW; i += BS) sh.cnt[i] = 0;
 
   // per-XB (thin, face) slot with the target-valid flag; displacement origins
   for (int j = tid; j < X; j += BS)
@@ -1429,6 +1427,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
 
       // ---- K2: XB transitions + binding bookkeeping (transitions.py:1081-1245)
       float m_atpe = 0.0f, m_tear = 0.0f;
+      int c_xb[6] = {0, 0, 0, 0, 0, 0}, c_atp = 0, c_new = 0;  // wave-uniform counts
 #pragma unroll
       for (int q = 0; q < MX; ++q) {
         const int j = tid + q * BS;
@@ -1469,9 +1468,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         // state counts of the new XB states and per-step transition counts
 #pragma unroll
-        for (int k = 0; k < 6; ++k) wave_count(&sh.cnt[C_XB0 + k], on && nw == k, lane);
-        wave_count(&sh.cnt[C_ATPC], on && old == 3 && (nw == 4 || nw == 0), lane);
-        wave_count(&sh.cnt[C_NEWLY], on && old == 0 && nw == 1, lane);
+        for (int k = 0; k < 6; ++k) c_xb[k] += wave_count(on && nw == k);
+        c_atp += wave_count(on && old == 3 && (nw == 4 || nw == 0));
+        c_new += wave_count(on && old == 0 && nw == 1);
+      }
+      if (lane == 0) {
+        LDS int* c = sh.cnt + wave * NC;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[C_XB0 + k] = c_xb[k];
+        c[C_ATPC] = c_atp;
+        c[C_NEWLY] = c_new;
       }
       {
         const float ae = wave_sum(m_atpe, lane), te = wave_sum(m_tear, lane);
@@ -1562,6 +1568,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         wp[W_SQK] = qk; wp[W_SQN] = qn;
       }
       // TM state and overlap counts on the post-solve positions
+      int c_tm[4] = {0, 0, 0, 0}, c_ov = 0, c_ov2 = 0, c_ov3 = 0;  // wave-uniform
       for (int s2 = tid; s2 < S + (BS - S % BS) % BS; s2 += BS) {
         const bool on = s2 < S;
         const int st = on ? tmc_p[s2] : -1;
@@ -1571,15 +1578,23 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           ov = x >= ov_lo && x <= ov_hi && x > 0.0f;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) wave_count(&sh.cnt[C_TM0 + k], st == k, lane);
-        wave_count(&sh.cnt[C_OV], ov, lane);
-        wave_count(&sh.cnt[C_OV2], ov && st == 2, lane);
-        wave_count(&sh.cnt[C_OV3], ov && st == 3, lane);
+        for (int k = 0; k < 4; ++k) c_tm[k] += wave_count(st == k);
+        c_ov += wave_count(ov);
+        c_ov2 += wave_count(ov && st == 2);
+        c_ov3 += wave_count(ov && st == 3);
+      }
+      if (lane == 0) {
+        LDS int* c = sh.cnt + wave * NC;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[C_TM0 + k] = c_tm[k];
+        c[C_OV] = c_ov;
+        c[C_OV2] = c_ov2;
+        c[C_OV3] = c_ov3;
       }
       __syncthreads();
       if (wave == 0) {
         // lane k < NWP folds wave-partial column k, lanes 16.. fold the
-        // per-filament columns, lanes 32.. take (and reset) the counters;
+        // per-filament columns, lanes 32.. fold the per-wave counters;
         // lane 0 then reads them across the wave and writes the row.
         double v = 0.0;
         if (lane < NWP) {
@@ -1592,8 +1607,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         } else if (lane >= 16 && lane < 16 + NF_SUM) {
           for (int f = 0; f < dm.Tk; ++f) v += sh.fil[f * NF + (lane - 16)];
         } else if (lane >= 32 && lane < 32 + NC) {
-          v = (double)sh.cnt[lane - 32];
-          sh.cnt[lane - 32] = 0;
+          int c = 0;
+          for (int w = 0; w < NW; ++w) c += sh.cnt[w * NC + (lane - 32)];
+          v = (double)c;
         }
         const double force = read_lane(v, 16 + F_FORCE);
         // population variances (jnp.std) from the sums of squares, in double
@@ -1644,9 +1660,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
         }
       }
-    } else if (tid == 0) {
-#pragma unroll
-      for (int k = 0; k < NC; ++k) sh.cnt[k] = 0;
     }
   }
 
