@@ -943,9 +943,11 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       if (!(g > eta2 * gamma0)) break;
       // r.u shrinks by a near-constant factor per pass: predict the next one
       const bool last = eta2 > 1e-14f && it > 0 && g * g <= eta2 * gamma0 * gamma;
-      const float beta = it == 0 ? 0.0f : g / (gamma + 1e-30f);
-      const float den = it == 0 ? dl : dl - beta * g / alpha;
-      alpha = g / (den + 1e-30f);
+      // the recurrence coefficients take the hardware reciprocal (1 ulp):
+      // CG only has to produce a correction, the FP64 residual checks it
+      const float beta = it == 0 ? 0.0f : g * __builtin_amdgcn_rcpf(gamma + 1e-30f);
+      const float den = it == 0 ? dl : dl - beta * g * __builtin_amdgcn_rcpf(alpha);
+      alpha = g * __builtin_amdgcn_rcpf(den + 1e-30f);
       gamma = g;
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
