@@ -24,6 +24,9 @@ using namespace mf;
 // instance's XB sampling and absorbing-chain rows are copied into LDS after
 // its other arrays.
 constexpr int LAUNCH_TABLES_IN_LDS = 1 << 16;
+// Internal launch flag: float positions of every (thin, face) candidate list
+// in LDS for the nearest-site search (set when they fit).
+constexpr int LAUNCH_FACE_POS = 1 << 17;
 
 // Pointers into the workgroup's LDS carry the local address space so every
 // access compiles to ds_* instructions (a generic pointer would go through
@@ -1097,7 +1100,8 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
 constexpr int NEAR_CHUNK = 8;
 template <bool REF64, bool GW>
 __device__ __forceinline__ void nearest_site(const Shared<REF64, GW>& sh, const Dims& dm, int slot, int thin,
-                                             int crown, int& nearest, float& xdist) {
+                                             int crown, int& nearest, float& xdist, const LDS float* fp,
+                                             int Wp) {
   const float base = posf<REF64>(sh, crown);
   const float head = base + XB_REACH;
   const int cnt = sh.fcount[slot];
@@ -1105,6 +1109,23 @@ __device__ __forceinline__ void nearest_site(const Shared<REF64, GW>& sh, const 
   const int tb = dm.nk + thin * dm.ns;
   float best = INFINITY;
   int bj = 0;
+  if (fp) {
+    // the face's candidate positions, masked entries already +inf
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const LDS f4* row = (const LDS f4*)(fp + slot * Wp);
+#pragma unroll 2
+    for (int q = 0; q < Wp / 4; ++q) {
+      const f4 v = row[q];
+      const float d0 = fabsf(v.x - head), d1 = fabsf(v.y - head), d2 = fabsf(v.z - head), d3 = fabsf(v.w - head);
+      if (d0 < best) { best = d0; bj = 4 * q; }
+      if (d1 < best) { best = d1; bj = 4 * q + 1; }
+      if (d2 < best) { best = d2; bj = 4 * q + 2; }
+      if (d3 < best) { best = d3; bj = 4 * q + 3; }
+    }
+    nearest = cs[bj];
+    xdist = posf<REF64>(sh, tb + nearest) - base;
+    return;
+  }
   for (int j0 = 0; j0 < dm.W; j0 += NEAR_CHUNK) {
     float sp[NEAR_CHUNK];
 #pragma unroll
@@ -1118,6 +1139,25 @@ __device__ __forceinline__ void nearest_site(const Shared<REF64, GW>& sh, const 
   }
   nearest = cs[bj];
   xdist = posf<REF64>(sh, tb + nearest) - base;
+}
+
+// Float positions of each (thin, face) candidate list, row stride Wp (W
+// rounded up to 4): the sites nearest_site compares, with padding slots and
+// sites at x <= 0 (geometry.py:113-116) stored as +inf so the search needs
+// no masks.
+template <bool REF64, bool GW>
+__device__ __forceinline__ void face_positions(const Shared<REF64, GW>& sh, const Dims& dm, LDS float* fp, int Wp,
+                                               int tid, int nthreads) {
+  const int n = dm.Tn * dm.nf * Wp;
+  for (int e = tid; e < n; e += nthreads) {
+    const int slot = e / Wp, j = e - slot * Wp;
+    float v = INFINITY;
+    if (j < sh.fcount[slot]) {
+      const float x = posf<REF64>(sh, dm.nk + (slot / dm.nf) * dm.ns + sh.fsites[slot * dm.W + j]);
+      v = x > 0.0f ? x : INFINITY;
+    }
+    fp[e] = v;
+  }
 }
 
 // clip(digitize(x, edges) - 1, 0, nb - 1) (transitions.py:915-917): the
@@ -1299,14 +1339,20 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   const uint8_t* xlab = dm.kxb > 1 ? L.pop_labels + I.label_off : nullptr;
   const uint8_t* tlab = dm.ktm > 1 ? L.pop_labels + I.label_off + X : nullptr;
   const Pops pops = pops_of(L, b);
+  int64_t lds_extra = lds_offsets<BS, REF64, GW>(dm).total;
   if (L.launch_flags & LAUNCH_TABLES_IN_LDS) {
-    LDS float* tl = (LDS float*)(smem + lds_offsets<BS, REF64, GW>(dm).total);
+    LDS float* tl = (LDS float*)(smem + lds_extra);
     const int nt = dm.kxb * (2 * nb * 48 + 2 * nb * 4);
     for (int i = tid; i < nt; i += BS) tl[i] = tab[TAB_XB + i];
     xbtab = (float*)tl;
     abstab = (float*)(tl + dm.kxb * 2 * nb * 48);
+    lds_extra += 4LL * nt;
     __syncthreads();
   }
+  // candidate-site positions for the nearest-site search (when they fit)
+  const int Wp = (dm.W + 3) & ~3;
+  LDS float* fpos =
+      (L.launch_flags & LAUNCH_FACE_POS) ? (LDS float*)(smem + ((lds_extra + 15) & ~int64_t(15))) : nullptr;
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
 
   // ---- time loop; step -1 is create_and_equilibrate (simulation.py:633-642)
@@ -1321,10 +1367,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     if (!kin) {
       // Nearest update, then a solve against the parameter row's z_line and
       // lattice_spacing (the reference's run() leaves those at their defaults).
+      if (fpos) {
+        face_positions(sh, dm, fpos, Wp, tid, BS);
+        __syncthreads();
+      }
       for (int j = tid; j < X; j += BS) {
         int nr; float xd;
         const int slot = sh.xslot[j] & 0x7fff;
-        nearest_site<REF64>(sh, dm, slot, slot / dm.nf, j / dm.nx, nr, xd);
+        nearest_site<REF64>(sh, dm, slot, slot / dm.nf, j / dm.nx, nr, xd, fpos, Wp);
         // tm == 0 everywhere: permissiveness 0
         sh.xkey[j] = (short)((xlab ? xlab[j] * 2 * nb : 0) + bin_of(sh.edges, nb, inv_w, xd));
       }
@@ -1347,6 +1397,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         for (int i = dm.nk + tid; i < dm.N; i += BS) {
           if constexpr (REF64) sh.x64[i] += (double)dz; else sh.x32[i] += dz;
         }
+        __syncthreads();
+      }
+      if (fpos) {
+        face_positions(sh, dm, fpos, Wp, tid, BS);
         __syncthreads();
       }
 
@@ -1391,7 +1445,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           float xd;
           const int slot = sh.xslot[j] & 0x7fff;
           const int thin = slot / dm.nf;
-          nearest_site<REF64>(sh, dm, slot, thin, j / dm.nx, q_near[q], xd);
+          nearest_site<REF64>(sh, dm, slot, thin, j / dm.nx, q_near[q], xd, fpos, Wp);
           q_near[q] += thin * dm.ns;  // thin-global site index from here on
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
           const int lb = xlab ? xlab[j] * 2 * nb : 0;
@@ -1883,10 +1937,19 @@ int mf_run(const mf_launch* L, void* stream) {
   for (const auto& G : lats)
     tab_bytes = std::max<int64_t>(tab_bytes, 4LL * kxb * (2LL * G.n_bins * 48 + 2LL * G.n_bins * 4));
   mf_launch Lk = *L;
-  Lk.launch_flags &= ~LAUNCH_TABLES_IN_LDS;
+  Lk.launch_flags &= ~(LAUNCH_TABLES_IN_LDS | LAUNCH_FACE_POS);
   if (lds + tab_bytes <= 160 * 1024) {
     Lk.launch_flags |= LAUNCH_TABLES_IN_LDS;
     lds += tab_bytes;
+  }
+  // then the candidate-site positions of the nearest-site search
+  int64_t fpos_bytes = 0;
+  for (const auto& G : lats)
+    fpos_bytes = std::max<int64_t>(fpos_bytes, 4LL * G.n_thin * G.n_faces * ((G.width + 3) & ~3));
+  fpos_bytes = 16 + ((fpos_bytes + 15) & ~int64_t(15));  // + alignment slack of the kernel's offset
+  if (lds + fpos_bytes <= 160 * 1024) {
+    Lk.launch_flags |= LAUNCH_FACE_POS;
+    lds += fpos_bytes;
   }
   HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   void* args[] = {(void*)&Lk};
