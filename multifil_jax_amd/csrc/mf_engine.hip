@@ -1356,10 +1356,18 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   const float* U = L.uniforms ? L.uniforms + (int64_t)b * L.t_stride * L.u_stride : nullptr;
 
   // ---- time loop; step -1 is create_and_equilibrate (simulation.py:633-642)
+  // drivers are read one step ahead, so that the load has landed by the
+  // time the step needs them; z_prev is the previous step's z_line
+  float z_prev = 0.0f, d_z = 0.0f, d_p = 0.0f, d_l = 0.0f;
+  if (init || n_steps > 0) { d_z = drv[t_first * 3 + 0]; d_p = drv[t_first * 3 + 1]; d_l = drv[t_first * 3 + 2]; }
+  if (n_steps > 0 && t_first > 0) z_prev = drv[(t_first - 1) * 3];
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
     const bool kin = step >= 0;
     const int t = t_first + (kin ? step : 0);
-    const float zl = drv[t * 3 + 0], pCa = drv[t * 3 + 1], lsd = drv[t * 3 + 2];
+    const float zl = d_z, pCa = d_p, lsd = d_l;
+    if (kin && step + 1 < n_steps) {
+      d_z = drv[(t + 1) * 3 + 0]; d_p = drv[(t + 1) * 3 + 1]; d_l = drv[(t + 1) * 3 + 2];
+    }
     float z_solve, d_solve;
     // old key and state of each head, for the dynamic-spacing metrics (their
     // absorbing chain is evaluated at the solved spacing)
@@ -1382,7 +1390,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       z_solve = sh.prm[P_ZLINE];
       d_solve = sh.prm[P_LS];
     } else {
-      const float dz = t > 0 ? zl - drv[(t - 1) * 3 + 0] : 0.0f;
+      const float dz = t > 0 ? zl - z_prev : 0.0f;
+      z_prev = zl;
       z_solve = zl;
       d_solve = lsd;
       if (DYN && rad.on) {
@@ -1648,72 +1657,99 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         c[C_OV3] = c_ov3;
       }
       __syncthreads();
+      // The cross-wave fold is split over waves 0-2, which run it
+      // concurrently: lane k folds one column (across the waves' partials or
+      // the filaments) and lane 0 writes the wave's part of the metrics row.
+      float* mrow = L.metrics ? L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS : nullptr;
       if (wave == 0) {
-        // lane k < NWP folds wave-partial column k, lanes 16.. fold the
-        // per-filament columns, lanes 32.. fold the per-wave counters;
-        // lane 0 then reads them across the wave and writes the row.
-        double v = 0.0;
-        if (lane < NWP) {
-          const bool ismax = lane == W_MAXK || lane == W_NMINK || lane == W_MAXN || lane == W_NMINN;
-          v = ismax ? -INFINITY : 0.0;
-          for (int w = 0; w < NW; ++w) {
-            const double x = sh.wpart[w * NWP + lane];
-            v = ismax ? fmax(v, x) : v + x;
-          }
-        } else if (lane >= 16 && lane < 16 + NF_SUM) {
-          for (int f = 0; f < dm.Tk; ++f) v += sh.fil[f * NF + (lane - 16)];
-        } else if (lane >= 32 && lane < 32 + NC) {
-          int c = 0;
-          for (int w = 0; w < NW; ++w) c += sh.cnt[w * NC + (lane - 32)];
-          v = (double)c;
+        // node moments and the XB expectations; lanes past NWP fold column 0
+        // again so that every lane's loads are unconditional
+        const int col = lane < NWP ? lane : 0;
+        const bool ismax = col == W_MAXK || col == W_NMINK || col == W_MAXN || col == W_NMINN;
+        double v = ismax ? -INFINITY : 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const double x = sh.wpart[w * NWP + col];
+          v = ismax ? fmax(v, x) : v + x;
         }
-        const double force = read_lane(v, 16 + F_FORCE);
         // population variances (jnp.std) from the sums of squares, in double
         // (displacements are O(100) nm at most: the cancellation costs a few
         // of double's 16 digits, far from the float32 result)
         const double mean_k = read_lane(v, W_SUMK) * inv_nk, mean_n = read_lane(v, W_SUMN) * inv_S;
         const double var_k = fmax(read_lane(v, W_SQK) * inv_nk - mean_k * mean_k, 0.0);
         const double var_n = fmax(read_lane(v, W_SQN) * inv_S - mean_n * mean_n, 0.0);
-        const double work = force * ((read_lane(v, W_WORK) - read_lane(v, W_XPRE)) * inv_nk);
-        const double atpe = read_lane(v, W_ATPE);
+        // every lane reads the folded values (cross-lane reads stay outside
+        // the lane-0 branch, where the other lanes' registers are not live)
+        const double mxk = read_lane(v, W_MAXK), nmk = read_lane(v, W_NMINK);
+        const double mxn = read_lane(v, W_MAXN), nmn = read_lane(v, W_NMINN);
+        const double atpe = read_lane(v, W_ATPE), tear = read_lane(v, W_TEAR);
+        if (lane == 0 && mrow) {
+          float* m = mrow;
+          m[34] = (float)mean_k; m[35] = (float)mxk; m[36] = (float)(-nmk);
+          m[37] = (float)sqrt(var_k);
+          m[38] = (float)mean_n; m[39] = (float)mxn; m[40] = (float)(-nmn);
+          m[41] = (float)sqrt(var_n);
+          m[48] = (float)atpe; m[49] = (float)tear;
+        }
+      } else if (wave == 1) {
+        // force and energies (lanes < NF_SUM: per-filament columns), work
+        // and expected ATP (lanes 8-10: wave-partial columns)
+        double v = 0.0;
+        if (lane < NF_SUM) {
+          for (int f = 0; f < dm.Tk; ++f) v += sh.fil[f * NF + lane];
+        } else if (lane >= 8 && lane < 11) {
+          const int col = lane == 8 ? W_WORK : lane == 9 ? W_XPRE : W_ATPE;
+#pragma unroll
+          for (int w = 0; w < NW; ++w) v += sh.wpart[w * NWP + col];
+        }
+        const double force = read_lane(v, F_FORCE);
+        const double work = force * ((read_lane(v, 8) - read_lane(v, 9)) * inv_nk);
+        const double atpe = read_lane(v, 10);
+        const double e1 = read_lane(v, F_E1), de1 = read_lane(v, F_DE1);
+        const double tit = read_lane(v, F_TIT), dtit = read_lane(v, F_DTIT);
         if (lane == 0) {
-          if (L.metrics) {
-            float c[NC];
-#pragma unroll
-            for (int k = 0; k < NC; ++k) c[k] = (float)read_lane(v, 32 + k);
-            const float nbound = c[C_XB0 + 1] + c[C_XB0 + 2] + c[C_XB0 + 3];
-            const float ov = c[C_OV];
-            const float invX = 1.0f / (float)X, invS = 1.0f / (float)S, invTk = 1.0f / (float)dm.Tk;
-            float* m = L.metrics + ((int64_t)b * L.t_stride + t) * MF_N_METRICS;
+          if (mrow) {
+            float* m = mrow;
+            const float invTk = 1.0f / (float)dm.Tk;
             m[0] = (float)force; m[1] = resid; m[2] = zl; m[3] = pCa; m[4] = (float)ls_new;
-            m[5] = nbound;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) m[6 + k] = c[C_XB0 + k];
-            m[12] = nbound * invX;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) m[13 + k] = c[C_XB0 + k] * invX;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) m[19 + k] = c[C_TM0 + k];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) m[23 + k] = c[C_TM0 + k] * invS;
-            m[27] = c[C_TM0 + 3] * invS;
-            m[28] = c[C_OV2] / ov;
-            m[29] = c[C_OV3] / ov;
-            m[30] = (c[C_OV2] + c[C_OV3]) / ov;
-            m[31] = ov;
-            m[32] = c[C_ATPC]; m[33] = c[C_NEWLY];
-            m[34] = (float)mean_k; m[35] = (float)read_lane(v, W_MAXK); m[36] = (float)(-read_lane(v, W_NMINK));
-            m[37] = (float)sqrt(var_k);
-            m[38] = (float)mean_n; m[39] = (float)read_lane(v, W_MAXN); m[40] = (float)(-read_lane(v, W_NMINN));
-            m[41] = (float)sqrt(var_n);
-            m[42] = (float)read_lane(v, 16 + F_E1) * invTk; m[43] = (float)read_lane(v, 16 + F_DE1) * invTk;
-            m[44] = (float)read_lane(v, 16 + F_TIT) * invTk; m[45] = (float)read_lane(v, 16 + F_DTIT) * invTk;
+            m[42] = (float)e1 * invTk; m[43] = (float)de1 * invTk;
+            m[44] = (float)tit * invTk; m[45] = (float)dtit * invTk;
             m[46] = (float)work; m[47] = (float)(work * inv_nk);
-            m[48] = (float)atpe; m[49] = (float)read_lane(v, W_TEAR);
             m[50] = (float)(atpe > 0.01 ? work / atpe : 0.0);
             m[51] = (float)iters;
           }
           if (L.force) L.force[(int64_t)b * L.t_stride + t] = force;
+        }
+      } else if (wave == 2 && mrow) {
+        // state and overlap counts (lane k < NC: counter k)
+        const int col = lane < NC ? lane : 0;
+        int cv = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) cv += sh.cnt[w * NC + col];
+        float c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) c[k] = (float)__builtin_amdgcn_readlane(cv, k);
+        if (lane == 0) {
+          const float nbound = c[C_XB0 + 1] + c[C_XB0 + 2] + c[C_XB0 + 3];
+          const float ov = c[C_OV];
+          const float invX = 1.0f / (float)X, invS = 1.0f / (float)S;
+          float* m = mrow;
+          m[5] = nbound;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) m[6 + k] = c[C_XB0 + k];
+          m[12] = nbound * invX;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) m[13 + k] = c[C_XB0 + k] * invX;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) m[19 + k] = c[C_TM0 + k];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) m[23 + k] = c[C_TM0 + k] * invS;
+          m[27] = c[C_TM0 + 3] * invS;
+          m[28] = c[C_OV2] / ov;
+          m[29] = c[C_OV3] / ov;
+          m[30] = (c[C_OV2] + c[C_OV3]) / ov;
+          m[31] = ov;
+          m[32] = c[C_ATPC]; m[33] = c[C_NEWLY];
         }
       }
     }
