@@ -641,14 +641,22 @@ __device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, cons
       continue;
     }
     if (sl.thick[s]) {
+      // the crown's heads: all loads first (h past n_xb_per_crown reads
+      // head 0 and is discarded), and the node index kept from being
+      // hoisted out of the time loop (and spilled)
+      int n_ = n;
+      asm volatile("" : "+v"(n_));
+      constexpr int PQ = Partners<MS>::P;
+      int site[PQ], st[PQ];
 #pragma unroll
-      for (int h = 0; h < PM; ++h) {
-        if (h >= dm.nx) break;
-        const int j = n * dm.nx + h;
-        const int site = sh.xsite[j];
-        const int st = sh.xst[j];
-        if (site >= 0 && st >= 1 && st <= 3) pt.pk[s][h] = (site << 16) | j;
+      for (int h = 0; h < PQ; ++h) {
+        const int j = n_ * dm.nx + (h < dm.nx ? h : 0);
+        site[h] = sh.xsite[j];
+        st[h] = sh.xst[j];
       }
+#pragma unroll
+      for (int h = 0; h < PM; ++h)
+        if (h < dm.nx && site[h] >= 0 && st[h] >= 1 && st[h] <= 3) pt.pk[s][h] = (site[h] << 16) | (n_ * dm.nx + h);
     } else {
       int j = sh.shead[n - dm.nk];
 #pragma unroll
@@ -1361,7 +1369,15 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   float z_prev = 0.0f, d_z = 0.0f, d_p = 0.0f, d_l = 0.0f;
   if (init || n_steps > 0) { d_z = drv[t_first * 3 + 0]; d_p = drv[t_first * 3 + 1]; d_l = drv[t_first * 3 + 2]; }
   if (n_steps > 0 && t_first > 0) z_prev = drv[(t_first - 1) * 3];
+  const int tid_k = tid;
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
+    // The thread index is made opaque once per step: otherwise the compiler
+    // hoists every tid-derived LDS address of the strided per-head and
+    // per-site loops out of the time loop and, short of registers, spills
+    // them, paying a scratch round trip per use; recomputing an address is
+    // one VALU operation.
+    int tid = tid_k;
+    asm volatile("" : "+v"(tid));
     const bool kin = step >= 0;
     const int t = t_first + (kin ? step : 0);
     const float zl = d_z, pCa = d_p, lsd = d_l;
