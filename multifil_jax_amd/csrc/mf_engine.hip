@@ -833,12 +833,19 @@ __device__ __forceinline__ void neg_jvp(const Shared<REF64, GW>& sh, const Dims&
     const float p0 = pv[n];
     float q = 0.0f;  // sum over partners of k (p_partner - p0)
     constexpr int PM = Partners<MS>::PM;
+    // the partners' entries are gathered first, all loads in flight
+    // together; the empty asm consumes the first and last, so the compiler
+    // cannot interleave the loads with the sums (it would, to save
+    // registers, serialising four LDS round trips)
+    float pp[Partners<MS>::P];
 #pragma unroll
     for (int h = 0; h < PM; ++h) {
       const int pk = pt.pk[s][h];
-      const int pn = pk >= 0 ? p_node(pk) : n;
-      q += pt.k[s][h] * (pv[pn] - p0);
+      pp[h] = pv[pk >= 0 ? p_node(pk) : n];
     }
+    if constexpr (PM > 0) asm volatile("" ::"v"(pp[0]), "v"(pp[PM - 1]));
+#pragma unroll
+    for (int h = 0; h < PM; ++h) q += pt.k[s][h] * (pp[h] - p0);
     if (PM == 0 && sl.thick[s]) {
       // the crown's own heads (stiffness 0 for unbound ones)
       for (int h = 0; h < dm.nx; ++h) {
