@@ -1645,10 +1645,22 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           sn += dd; qn += dd * dd; mxn = fmax(mxn, dd); nmn = fmax(nmn, -dd);
         }
       }
-      sk = wave_sum(sk, lane); sn = wave_sum(sn, lane); wk = wave_sum(wk, lane);
-      qk = wave_sum(qk, lane); qn = wave_sum(qn, lane);
-      mxk = wave_max(mxk, lane); nmk = wave_max(nmk, lane);
-      mxn = wave_max(mxn, lane); nmn = wave_max(nmn, lane);
+      // A wave that owns no thick (or no thin) node skips that kind's
+      // reductions: every lane already holds the identity (0 or -inf).
+      bool any_k = false, any_n = false;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        any_k = any_k || (sl.node[s] >= 0 && sl.thick[s]);
+        any_n = any_n || (sl.node[s] >= 0 && !sl.thick[s]);
+      }
+      if (__ballot(any_k)) {
+        sk = wave_sum(sk, lane); wk = wave_sum(wk, lane); qk = wave_sum(qk, lane);
+        mxk = wave_max(mxk, lane); nmk = wave_max(nmk, lane);
+      }
+      if (__ballot(any_n)) {
+        sn = wave_sum(sn, lane); qn = wave_sum(qn, lane);
+        mxn = wave_max(mxn, lane); nmn = wave_max(nmn, lane);
+      }
       if (lane == 0) {
         LDS double* wp = sh.wpart + wave * NWP;
         wp[W_SUMK] = sk; wp[W_SUMN] = sn; wp[W_WORK] = wk;
