@@ -263,7 +263,7 @@ enum Counter { C_XB0 = 0, C_TM0 = 6, C_OV = 10, C_OV2, C_OV3, C_ATPC, C_NEWLY, N
 // Per-thick-filament metric values written by the owning lane; the first
 // NF_SUM are summed over filaments, F_X0 / F_XT hold the pre-solve positions
 // of the M-line and tip crowns.
-enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF_SUM, F_X0 = NF_SUM, F_XT, NF };
+enum FilVal { F_FORCE = 0, F_E1, F_DE1, F_TIT, F_DTIT, NF_SUM, F_X0 = NF_SUM, F_XT, F_EO, NF };
 // Per-wave partial sums: pass 1 (node sums / maxima, XB expectations), pass 2,
 // and the pre-solve sum of thick positions (work = sum x_new - sum x_pre).
 // W_RG / W_RGD carry the radial sums of a dynamic-spacing residual.
@@ -611,12 +611,16 @@ __device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T
 // Titin axial force on the tip crown and its derivative with respect to the
 // tip position (forces.py:136-157).
 template <typename T>
-__device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt) {
+__device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt, T& energy) {
   const T ad = z - tip;
   const T Lq = sqrt(ad * ad + d * d);
   const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
   const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
-  T F = T(prm[P_TITIN_A]) * exp(argc);
+  const T E = exp(argc);
+  // stored energy a/b (exp(b (L - rest)) - 1) (metrics_fn.py), from the
+  // unclamped exponent
+  energy = T(prm[P_TITIN_A]) / T(prm[P_TITIN_B]) * ((arg == argc ? E : exp(arg)) - T(1));
+  T F = T(prm[P_TITIN_A]) * E;
   F = F > T(0) ? F : T(0);
   const T cs = Lq > T(0) ? ad / Lq : T(0);
   tt = T(N_TITIN) * F * cs;
@@ -694,7 +698,8 @@ template <int BS, int MS, bool REF64, bool DYN, typename T, bool GW>
 __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
                                           const float (&rA)[MS], const float (&rB)[MS], T zl, T d,
                                           float (&f)[MS], float (&cb)[MS], Partners<MS>& pt,
-                                          Radial& rad, Red<BS>& rd, int tid, int lane, int wave) {
+                                          Radial& rad, Red<BS>& rd, int tid, int lane, int wave,
+                                          bool first) {
   const LDS float* prm = sh.prm;
   const bool dyn = DYN && rad.on;
   const auto fdx = sh.pv;  // [X] d f / d d per head, dynamic spacing only
@@ -762,8 +767,17 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
         s_hi = (T(posd<REF64>(sh, n + 1)) - x0 - T(rB[s])) * kk;
       } else {
         float kt;
-        titin<T>(prm, x0, zl, d, s_hi, kt);
+        T en;
+        titin<T>(prm, x0, zl, d, s_hi, kt, en);
         sh.ktip[sl.fil[s]] = kt;
+        if constexpr (REF64) {
+          // the tip's titin energy for the metrics: the step's first
+          // evaluation is at the pre-solve positions, its last at the solved
+          // ones (same lane, read back by it after the solve)
+          LDS double* fv = sh.fil + sl.fil[s] * NF;
+          fv[F_TIT] = (double)en;
+          if (first) fv[F_EO] = (double)en;
+        }
         if (dyn) {
           T tr;
           float trd, ttd;
@@ -1071,7 +1085,7 @@ __device__ __forceinline__ void solve(const Shared<REF64, GW>& sh, const Dims& d
   float mx, prev = INFINITY;
   for (;;) {
     mx = residual<BS, MS, REF64, DYN, T>(sh, dm, sl, rA, rB, T(zl), d, fb, cb, pt, rad, rd, tid, lane,
-                                         wave);
+                                         wave, it == 0);
     // the d-block preconditioner uses J_dd at the initial state (solver.py:633-638)
     if (DYN && rad.on && it == 0) rad.dinv = -1.0f / rad.jdd;
     if (!(mx > tol)) break;
@@ -1634,12 +1648,18 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             fv[F_DE1] = e1 - 0.5 * kk * d0 * d0;
           }
           if (sl.li[s] == dm.nc - 1) {
-            const double an = (double)zl - x, ao = (double)zl - fv[F_XT];
-            const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
-            const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
-            const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
-            fv[F_TIT] = en;
-            fv[F_DTIT] = en - eo;
+            if (REF64 && !(DYN && rad.on)) {
+              // the solve evaluated both at this Z-line and spacing: F_TIT
+              // holds the energy at the solved tip, F_EO at the pre-solve one
+              fv[F_DTIT] = fv[F_TIT] - fv[F_EO];
+            } else {
+              const double an = (double)zl - x, ao = (double)zl - fv[F_XT];
+              const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
+              const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
+              const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
+              fv[F_TIT] = en;
+              fv[F_DTIT] = en - eo;
+            }
           }
         } else {
           sn += dd; qn += dd * dd; mxn = fmax(mxn, dd); nmn = fmax(nmn, -dd);
