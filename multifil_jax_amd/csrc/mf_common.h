@@ -62,6 +62,31 @@ __device__ __forceinline__ float philox_uniform(uint64_t seed, int step, int pur
   return (float)(bits >> 9) * 1.1920928955078125e-07f;
 }
 
+// philox_uniform(seed, step, 0, idx) and philox_uniform(seed, step, 1, idx)
+// for idx = the lane's thread index, at one Philox block per lane instead of
+// two: in each quad (idx 4m..4m+3, which share block m) lanes 0-1 compute
+// block m of purpose 0 and lanes 2-3 block m of purpose 1, and every lane
+// takes its word idx%4 of each by a quad broadcast from lanes 0 and 2.
+// All four lanes of every quad must be active.
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ void philox_uniform_pair(uint64_t seed, int step, int idx, float& u0, float& u1) {
+  const int r = idx & 3;
+  const uint4 w = philox4x32_10(make_uint4((uint32_t)(idx >> 2), (uint32_t)step, r < 2 ? 0u : 1u, 0u),
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+  // quad_perm(0,0,0,0) = 0x00 and quad_perm(2,2,2,2) = 0xAA
+  const uint32_t a0 = quad_bcast<0x00>(w.x), a1 = quad_bcast<0x00>(w.y), a2 = quad_bcast<0x00>(w.z),
+                 a3 = quad_bcast<0x00>(w.w);
+  const uint32_t b0 = quad_bcast<0xAA>(w.x), b1 = quad_bcast<0xAA>(w.y), b2 = quad_bcast<0xAA>(w.z),
+                 b3 = quad_bcast<0xAA>(w.w);
+  const uint32_t x0 = r == 0 ? a0 : r == 1 ? a1 : r == 2 ? a2 : a3;
+  const uint32_t x1 = r == 0 ? b0 : r == 1 ? b1 : r == 2 ? b2 : b3;
+  u0 = (float)(x0 >> 9) * 1.1920928955078125e-07f;
+  u1 = (float)(x1 >> 9) * 1.1920928955078125e-07f;
+}
+
 // ---------------------------------------------------------------------------
 // wave-level primitives (wave64, DPP)
 //
