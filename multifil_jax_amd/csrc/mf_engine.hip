@@ -1391,6 +1391,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   if (init || n_steps > 0) { d_z = drv[t_first * 3 + 0]; d_p = drv[t_first * 3 + 1]; d_l = drv[t_first * 3 + 2]; }
   if (n_steps > 0 && t_first > 0) z_prev = drv[(t_first - 1) * 3];
   const int tid_k = tid;
+  bool fpos_ready = false;  // fpos already holds this step's candidate positions
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
     // The thread index is made opaque once per step: otherwise the compiler
     // hoists every tid-derived LDS address of the strided per-head and
@@ -1445,10 +1446,11 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         __syncthreads();
       }
-      if (fpos) {
+      if (fpos && !fpos_ready) {
         face_positions(sh, dm, fpos, Wp, tid, BS);
         __syncthreads();
       }
+      fpos_ready = false;
 
       // ---- K1: TM transitions (transitions.py:516-625) + nearest sites -------
       const LDS unsigned char* tmo = sh.tm + tmc * S;
@@ -1716,6 +1718,11 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         c[C_OV2] = c_ov2;
         c[C_OV3] = c_ov3;
       }
+      // The next step's candidate positions, when its Z-line equals this
+      // one: then nothing moves the thin filaments before its search, and
+      // the positions are ready behind this barrier instead of one of its own.
+      fpos_ready = fpos != nullptr && step + 1 < n_steps && d_z == zl;
+      if (fpos_ready) face_positions(sh, dm, fpos, Wp, tid, BS);
       __syncthreads();
       // The cross-wave fold is split over waves 0-2, which run it
       // concurrently: lane k folds one column (across the waves' partials or
