@@ -144,6 +144,17 @@ __device__ __forceinline__ T wave_prefix_excl(T incl, int lane) {
   (void)lane;
   return dpp_shift<0x138, 0xf>(incl, T(0));  // wave_shr:1; lane 0 has no source
 }
+// Reduction over each 16-lane row (row_shr:1/2/4/8), total in the row's
+// lane 15.
+template <typename T, typename Op = OpAdd>
+__device__ __forceinline__ T row_reduce16(T v, Op op = Op()) {
+  const T id = Op::template identity<T>();
+  v = op(v, dpp_shift<0x111, 0xf>(v, id));
+  v = op(v, dpp_shift<0x112, 0xf>(v, id));
+  v = op(v, dpp_shift<0x114, 0xf>(v, id));
+  v = op(v, dpp_shift<0x118, 0xf>(v, id));
+  return v;
+}
 // wave-uniform reductions
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v, int lane) { return read_lane(wave_prefix(v, lane, OpAdd()), 63); }

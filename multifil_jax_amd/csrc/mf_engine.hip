@@ -416,58 +416,96 @@ struct Red {
   int ring;
   __device__ LDS float* row() { LDS float* p = buf + (ring & 7) * NW; ++ring; return p; }
 };
-template <int BS>
+// Cross-wave step of these reductions, after the barrier.  TREE = false:
+// every wave adds the NW partials in wave order.  TREE = true: lane w of row
+// q (w < NW <= 16) reads wave w's partial of quantity q and a 16-lane DPP
+// tree per row folds them (fewer instructions, a longer dependent chain:
+// it pays in the many-slot variants, not at two slots per lane).  Either
+// way every wave runs the same operations on the same partials and gets the
+// same bits.
+template <int BS, bool TREE = false>
 __device__ __forceinline__ float wg_sum(Red<BS>& rd, float v, int lane, int wave) {
+  static_assert(Red<BS>::NW <= 16, "one 16-lane row per quantity");
   LDS float* r = rd.row();
   const float s = wave_sum(v, lane);
   if (lane == 0) r[wave] = s;
   __syncthreads();
-  float t = 0.0f;
+  if constexpr (TREE) {
+    const float x = lane < Red<BS>::NW ? r[lane] : 0.0f;
+    return read_lane(row_reduce16(x), 15);
+  } else {
+    float t = 0.0f;
 #pragma unroll
-  for (int w = 0; w < Red<BS>::NW; ++w) t += r[w];
-  return t;
+    for (int w = 0; w < Red<BS>::NW; ++w) t += r[w];
+    return t;
+  }
 }
 // Two sums with one barrier (consumes two ring rows; a row is rewritten only
 // after another reduction's barrier, by which time every wave has read it).
-template <int BS>
+template <int BS, bool TREE = false>
 __device__ __forceinline__ void wg_sum2(Red<BS>& rd, float& a, float& b, int lane, int wave) {
+  static_assert(Red<BS>::NW <= 16, "one 16-lane row per quantity");
   LDS float* ra = rd.row();
   LDS float* rb = rd.row();
   const float sa = wave_sum(a, lane);
   const float sb = wave_sum(b, lane);
   if (lane == 0) { ra[wave] = sa; rb[wave] = sb; }
   __syncthreads();
-  float ta = 0.0f, tb = 0.0f;
+  if constexpr (TREE) {
+    const int w = lane & 15;
+    const float x = w < Red<BS>::NW && lane < 32 ? (lane < 16 ? ra[w] : rb[w]) : 0.0f;
+    const float t = row_reduce16(x);
+    a = read_lane(t, 15);
+    b = read_lane(t, 31);
+  } else {
+    float ta = 0.0f, tb = 0.0f;
 #pragma unroll
-  for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; }
-  a = ta;
-  b = tb;
+    for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; }
+    a = ta;
+    b = tb;
+  }
 }
-template <int BS>
+template <int BS, bool TREE = false>
 __device__ __forceinline__ void wg_sum3(Red<BS>& rd, float& a, float& b, float& c, int lane, int wave) {
+  static_assert(Red<BS>::NW <= 16, "one 16-lane row per quantity");
   LDS float* ra = rd.row();
   LDS float* rb = rd.row();
   LDS float* rc = rd.row();
   const float sa = wave_sum(a, lane), sb = wave_sum(b, lane), sc = wave_sum(c, lane);
   if (lane == 0) { ra[wave] = sa; rb[wave] = sb; rc[wave] = sc; }
   __syncthreads();
-  float ta = 0.0f, tb = 0.0f, tc = 0.0f;
+  if constexpr (TREE) {
+    const int w = lane & 15;
+    const float x = w < Red<BS>::NW && lane < 48 ? (lane < 16 ? ra[w] : lane < 32 ? rb[w] : rc[w]) : 0.0f;
+    const float t = row_reduce16(x);
+    a = read_lane(t, 15);
+    b = read_lane(t, 31);
+    c = read_lane(t, 47);
+  } else {
+    float ta = 0.0f, tb = 0.0f, tc = 0.0f;
 #pragma unroll
-  for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; tc += rc[w]; }
-  a = ta;
-  b = tb;
-  c = tc;
+    for (int w = 0; w < Red<BS>::NW; ++w) { ta += ra[w]; tb += rb[w]; tc += rc[w]; }
+    a = ta;
+    b = tb;
+    c = tc;
+  }
 }
-template <int BS>
+template <int BS, bool TREE = false>
 __device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave) {
+  static_assert(Red<BS>::NW <= 16, "one 16-lane row per quantity");
   LDS float* r = rd.row();
   const float s = wave_max(v, lane);
   if (lane == 0) r[wave] = s;
   __syncthreads();
-  float t = r[0];
+  if constexpr (TREE) {
+    const float x = lane < Red<BS>::NW ? r[lane] : -INFINITY;
+    return read_lane(row_reduce16(x, OpMax()), 15);
+  } else {
+    float t = r[0];
 #pragma unroll
-  for (int w = 1; w < Red<BS>::NW; ++w) t = fmaxf(t, r[w]);
-  return t;
+    for (int w = 1; w < Red<BS>::NW; ++w) t = fmaxf(t, r[w]);
+    return t;
+  }
 }
 
 // Scan of slot values along every chain of this wave.  dir 0: inclusive
@@ -827,7 +865,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
     rad.jdd = (float)(-T(rad.K) - gc);
     mx = fmaxf(mx, fabsf(rad.frad));
   }
-  return wg_max<BS>(rd, mx, lane, wave);
+  return wg_max<BS, (MS > 2)>(rd, mx, lane, wave);
 }
 
 // -J p for the slot nodes, p read from the shared vector buffer pv.
@@ -961,12 +999,12 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
       for (int s = 0; s < MS; ++s) { g += r[s] * u[s]; dl += w[s] * u[s]; }
       float w_d = 0.0f;
       if (dyn) {
-        wg_sum3<BS>(rd, g, dl, cu, lane, wave);
+        wg_sum3<BS, (MS > 2)>(rd, g, dl, cu, lane, wave);
         w_d = -cu - rad.jdd * u_d;
         g += r_d * u_d;
         dl += w_d * u_d;
       } else {
-        wg_sum2<BS>(rd, g, dl, lane, wave);
+        wg_sum2<BS, (MS > 2)>(rd, g, dl, lane, wave);
       }
       // converged to FP32 rounding (r.u ~ |r|^2 has fallen 14 orders) or
       // broken down: further passes only add rounding noise (every lane sees
@@ -1009,7 +1047,7 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
     float loc = 0.0f;
 #pragma unroll
     for (int s = 0; s < MS; ++s) loc += r[s] * z[s];
-    float rz_new = wg_sum<BS>(rd, loc, lane, wave);
+    float rz_new = wg_sum<BS, (MS > 2)>(rd, loc, lane, wave);
     if (dyn) rz_new += r_d * z_d;
     const float beta = it == 0 ? 0.0f : rz_new / (rz + 1e-30f);
     rz = rz_new;
@@ -1032,13 +1070,13 @@ __device__ __forceinline__ void pcg(const Shared<REF64, GW>& sh, const Dims& dm,
         cp += c * p[s];
         loc += p[s] * Ap[s];
       }
-      wg_sum2<BS>(rd, loc, cp, lane, wave);
+      wg_sum2<BS, (MS > 2)>(rd, loc, cp, lane, wave);
       Ap_d = -cp - rad.jdd * p_d;
       pAp = loc + p_d * Ap_d;
     } else {
 #pragma unroll
       for (int s = 0; s < MS; ++s) loc += p[s] * Ap[s];
-      pAp = wg_sum<BS>(rd, loc, lane, wave);
+      pAp = wg_sum<BS, (MS > 2)>(rd, loc, lane, wave);
     }
     const float alpha = rz / (pAp + 1e-30f);
 #pragma unroll
