@@ -1493,12 +1493,17 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       // ---- K1: TM transitions (transitions.py:516-625) + nearest sites -------
       const LDS unsigned char* tmo = sh.tm + tmc * S;
       LDS unsigned char* tmn = sh.tm + (tmc ^ 1) * S;
-      // device draws for one site and one head per thread: both uniforms
-      // from one Philox block per lane (every thread is active here)
-      const bool paired = U == nullptr && S <= BS && X <= BS;
-      float u_tm = 0.0f, u_xb = 0.0f;
-      if (paired) philox_uniform_pair(seed, t, tid, u_tm, u_xb);
-      for (int s = tid; s < S; s += BS) {
+      // device draws: when the sites and the heads both fit the MX strided
+      // slots of a thread, the site and head uniforms of slot k come from
+      // one Philox block per lane (every thread is active here)
+      const bool paired = U == nullptr && S <= MX * BS && X <= MX * BS;
+      float u_tm[MX], u_xb[MX];
+#pragma unroll
+      for (int k = 0; k < MX; ++k) {
+        u_tm[k] = u_xb[k] = 0.0f;
+        if (paired) philox_uniform_pair(seed, t, tid + k * BS, u_tm[k], u_xb[k]);
+      }
+      auto tm_site = [&](int s, float u) {
         const int pv = sh.tprev[s], nv = sh.tnext[s];
         const int sp_ = tmo[s + pv], sn_ = tmo[s + nv];
         const bool pr = pv != 0, nr = nv != 0;
@@ -1507,7 +1512,6 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         const int ncl = (pr && sp_ <= 1) + (nr && sn_ <= 1);
         const int cfg = 9 * n2 + 3 * n3 + ncl;
         const int st = tmo[s];
-        const float u = U ? U[(int64_t)t * L.u_stride + s] : paired ? u_tm : philox_uniform(seed, t, 0, s);
         int ns;
         if (st == 3 && sh.tbound[s] >= 0) {
           ns = 3;  // locked open under a bound head: [0, 0, 0, 1]
@@ -1517,6 +1521,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         tmn[s] = (unsigned char)ns;
         sh.shead[s] = -1;
+      };
+      if (paired) {
+#pragma unroll
+        for (int k = 0; k < MX; ++k) {
+          const int s = tid + k * BS;
+          if (s < S) tm_site(s, u_tm[k]);
+        }
+      } else {
+        for (int s = tid; s < S; s += BS)
+          tm_site(s, U ? U[(int64_t)t * L.u_stride + s] : philox_uniform(seed, t, 0, s));
       }
       // nearest sites, and the candidate table rows of both permissiveness
       // levels fetched ahead of the barrier (K2 only picks one)
@@ -1585,7 +1599,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const bool perm = (tmc_p[gsite] == 3) && (sh.xslot[j] < 0);
           const int key = (perm ? nb : 0) + q_bin[q];
           old = q_old[q];
-          const float u = U ? U[(int64_t)t * L.u_stride + S + j] : paired ? u_xb : philox_uniform(seed, t, 1, j);
+          const float u = U ? U[(int64_t)t * L.u_stride + S + j] : paired ? u_xb[q] : philox_uniform(seed, t, 1, j);
           const float4 c0 = perm ? q_r1[q] : q_r0[q];
           const float2 c1 = perm ? q_s1[q] : q_s0[q];
           nw = u < c0.x ? 0 : u < c0.y ? 1 : u < c0.z ? 2 : u < c0.w ? 3 : u < c1.x ? 4 : u < c1.y ? 5 : 0;
