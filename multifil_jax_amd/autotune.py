@@ -64,7 +64,7 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
     return out
 
 
-def tune(batch: Batch, mode: str = "refined", trial_steps: int = 40, warmup_steps: int = 20,
+def tune(batch: Batch, mode: str = "refined", trial_steps: int = 100, warmup_steps: int = 30,
          n_cg: Sequence[int] = N_CG, blocks: Optional[Sequence[int]] = None, max_instances: int = 256,
          verbose: bool = False, cg_forcing: Sequence[float] = CG_FORCING, repeats: int = 2):
     """Pick (block, n_cg, cg_forcing) for ``batch``; returns (block, SolverConfig).
