@@ -3,7 +3,7 @@
  *
  * The reference has no FFI: its hot path is pure-Python JAX.  The entry
  * points below replace these reference functions, and are what a ctypes
- * binding on the reference side would call (see INTEGRATION.md):
+ * binding on the reference side would call:
  *
  *   mf_tables  <- transitions.py:99-190  expm_pade6_batch, with the generator
  *                 builders transitions.py:434-513 (TM, 27 x 4x4) and

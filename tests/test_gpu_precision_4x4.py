@@ -10,7 +10,7 @@ Bars: identical discrete state; node positions within 1e-6 nm (1e-9
 relative to the ~1000 nm coordinates); M-line force within 1e-7 relative.
 Free run: 500 device steps with the same draws track the oracle's
 trajectory; up to the first step whose XB state counts differ the force
-stays within 1e-5 relative, and that prefix is at least
+stays within 1e-5 relative (the mean-force error bar), and that prefix is at least
 50 steps.  The prefix ends when a position, correct to the solver's 1e-5 pN
 residual (~1e-9 nm), falls on the other side of a float32 decision boundary
 (nearest site, bin edge) or a draw lands within rounding of a table
