@@ -268,11 +268,12 @@ def uneven_conditions(seed=20261203, per_lattice=96, lattices=(2, 3, 4, 6)):
     return lat, lengths, pCa
 
 
-def workload_uneven(cfg, block=None, autotune=True):
-    """96 instances each of 2x2 / 3x3 / 4x4 / 6x6, lengths U{200..1000},
-    dynamic spacing.  The LDS-resident lattices share one launch; 6x6 (device
-    workspace) runs in a second launch on its own stream, concurrently."""
-    import torch
+def uneven_batches(cfg, block=None, autotune=True):
+    """The uneven workload's two launches: 96 instances each of 2x2 / 3x3 /
+    4x4 / 6x6, lengths U{200..1000}, dynamic spacing.  The LDS-resident
+    lattices share one batch; 6x6 (device workspace) gets its own, run first
+    on its own stream.  Returns (batches, T, total sarcomere-steps); each
+    batch carries ``tuned_cfg``."""
     from multifil_jax_amd.topology import SarcTopology
     from multifil_jax_amd.params import get_skeletal_params
     from multifil_jax_amd.engine import LatticePack, Batch
@@ -299,21 +300,34 @@ def workload_uneven(cfg, block=None, autotune=True):
         K = np.array([5.0 * topos[i].n_thick for i in lat[idx]], np.float32)
         bt = Batch(LatticePack(gtopos), ids, d.rows(B), drv, seeds=idx.astype(np.uint64),
                    dynamic=(K, np.zeros(B, np.float32)), lengths=lengths[idx])
+        bt.tuned_cfg = cfg
         if block:
             bt.block = block
         elif autotune:
             from multifil_jax_amd.autotune import tune
-            bt.block, bt.tuned_cfg = tune(bt, mode=cfg.mode)
+            bt.block, bt.tuned_cfg = tune(bt, mode=cfg.mode, base=cfg)
         batches.append(bt)
+    return batches, T, int(lengths.sum())
+
+
+def run_uneven(batches, T):
+    """Both launches, concurrently on two streams; wall seconds."""
+    import torch
     streams = [torch.cuda.Stream() for _ in batches]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for bt, st in zip(batches, streams):
         with torch.cuda.stream(st):
-            bt.run(T, cfg=getattr(bt, "tuned_cfg", cfg), sync=False)
+            bt.run(T, cfg=bt.tuned_cfg, sync=False)
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    total = int(lengths.sum())
+    return time.perf_counter() - t0
+
+
+def workload_uneven(cfg, block=None, autotune=True):
+    """96 instances each of 2x2 / 3x3 / 4x4 / 6x6, lengths U{200..1000},
+    dynamic spacing, in two concurrent launches."""
+    batches, T, total = uneven_batches(cfg, block, autotune)
+    el = run_uneven(batches, T)
     worst = max(float(bt.metrics()[:, :, 1].max()) for bt in batches)
     return {"metric": "sarcomere-steps/s, uneven batch 384 x {2x2,3x3,4x4,6x6}, dynamic spacing",
             "value": total / el, "unit": "sarcomere-steps/s", "n_gpus": 1, "steps": T,
@@ -324,8 +338,8 @@ def workload_uneven(cfg, block=None, autotune=True):
                                    "z_line 900 nm, dynamic spacing K_lat 5 pN/nm per thick, nu 0, d0 14 nm, "
                                    "dt 1 ms; default_rng(20261203)",
                        "solver": cfg.mode, "launches": 2,
-                       "groups": [{"block": bt.block, "n_cg": getattr(bt, "tuned_cfg", cfg).n_cg,
-                                   "cg_forcing": getattr(bt, "tuned_cfg", cfg).cg_forcing,
+                       "groups": [{"block": bt.block, "n_cg": bt.tuned_cfg.n_cg,
+                                   "cg_forcing": bt.tuned_cfg.cg_forcing,
                                    "instances": bt.B} for bt in batches]}}
 
 
@@ -398,7 +412,7 @@ def main():
         bt.block = args.block
     elif not args.no_autotune:
         from multifil_jax_amd.autotune import tune
-        bt.block, cfg = tune(bt, mode=args.mode)
+        bt.block, cfg = tune(bt, mode=args.mode, base=cfg)
         tuned = {"block": bt.block, "n_cg": cfg.n_cg}
     # warmup: equilibration + W steps (also builds the transition tables)
     bt.run(W, t0=0, cfg=cfg)

@@ -15,12 +15,15 @@ The tuner runs short trial launches of the same instances on a copy of the
 batch. It tries every block size with a kernel variant for these lattices,
 times each (block, n_cg) pair with HIP events, and keeps the fastest one
 whose refined residual met its tolerance at every trial step. Choices are
-cached per (lattice shapes, solver mode, dynamic spacing, subpopulation
-mode) for the process.
+cached for the process per (lattice shapes, the fixed solver settings,
+dynamic spacing, subpopulation mode, batch size, a digest of the
+conditions).
 """
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
+import hashlib
 from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
@@ -35,10 +38,17 @@ CG_FORCING = (0.0, 0.003)
 _cache: Dict[tuple, Tuple[int, int, float]] = {}
 
 
-def _key(batch: Batch, mode: str) -> tuple:
+def _key(batch: Batch, base: SolverConfig) -> tuple:
+    """Cache key: lattice shapes, the solver settings the trials keep fixed,
+    dynamic spacing, subpopulations, the batch size and a digest of the
+    conditions (parameter rows, drivers, lengths, spacing stiffness)."""
     shapes = tuple(sorted({(t.n_thick, t.n_crowns, t.n_thin, t.n_sites) for t in batch.pack.topos}))
     pops = None if batch.pops is None else (batch.pops.mode, batch.pops.K, batch.pops.flags)
-    return shapes, mode, bool(batch.is_dynamic.any()), pops
+    h = hashlib.sha1()
+    for a in (batch.lattice_ids, batch.params_np, batch.drivers_np, batch.lengths, batch.K_lat, batch.nu):
+        h.update(np.ascontiguousarray(a).tobytes())
+    fixed = (base.mode, base.n_newton, base.n_refine, base.tol, base.refine_tol, base.dt)
+    return shapes, fixed, bool(batch.is_dynamic.any()), pops, batch.B, h.hexdigest()
 
 
 def feasible_blocks(batch: Batch, mode: str) -> list:
@@ -58,6 +68,10 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
             L.n_pops = batch.pops.K
             L.pop_mode = _lib.MF_POP_MEAN_FIELD if batch.pops.mode == "mean_field" else _lib.MF_POP_EXPLICIT
             L.pop_flags = batch.pops.flags
+            # the library checks that the population buffers a launch names exist
+            L.pop_params = batch.d_pop_params.data_ptr()
+            L.pop_fractions = batch.d_pop_frac.data_ptr()
+            L.pop_labels = batch.d_labels.data_ptr() if batch.d_labels is not None else None
         lds = C.c_int64(0)
         if lib.mf_run_lds_bytes(C.byref(L), C.byref(lds)) == 0:
             out.append(blk)
@@ -66,8 +80,13 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
 
 def tune(batch: Batch, mode: str = "refined", trial_steps: int = 100, warmup_steps: int = 30,
          n_cg: Sequence[int] = N_CG, blocks: Optional[Sequence[int]] = None, max_instances: int = 256,
-         verbose: bool = False, cg_forcing: Sequence[float] = CG_FORCING, repeats: int = 2):
+         verbose: bool = False, cg_forcing: Sequence[float] = CG_FORCING, repeats: int = 2,
+         base: Optional[SolverConfig] = None):
     """Pick (block, n_cg, cg_forcing) for ``batch``; returns (block, SolverConfig).
+
+    ``base`` carries the settings the trials keep (dt, Newton caps,
+    tolerances); it defaults to ``SolverConfig(mode=mode)``.  The returned
+    config is ``base`` with the tuned n_cg and cg_forcing.
 
     The trial copies up to ``max_instances`` instances (the longest-running
     ones when lengths differ), runs ``warmup_steps`` untimed steps past the
@@ -75,10 +94,11 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 100, warmup_ste
     is timed ``repeats`` times (interleaved passes) and scored by its best.
     The batch itself is left untouched."""
     import torch
-    key = _key(batch, mode)
+    base = SolverConfig(mode=mode) if base is None else dataclasses.replace(base, mode=mode)
+    key = _key(batch, base)
     if key in _cache:
         blk, ncg, frc = _cache[key]
-        return blk, SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)
+        return blk, dataclasses.replace(base, n_cg=ncg, cg_forcing=frc)
     if mode != "refined":
         # reference mode reproduces the reference's algorithm, whose CG count
         # (6, params.py:656) is part of it: only the launch shape is tuned
@@ -97,7 +117,7 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 100, warmup_ste
     times = {}
     cands = [(b_, n_, f_) for b_ in blocks for n_ in n_cg for f_ in cg_forcing]
     for blk, ncg, frc in cands * repeats:
-        cfg = SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)
+        cfg = dataclasses.replace(base, n_cg=ncg, cg_forcing=frc)
         trial = Batch(batch.pack, batch.lattice_ids[order], batch.params_np[order],
                       batch.drivers_np[order, :T], seeds=batch.seeds[order], dynamic=dyn,
                       pops=batch.pops.subset(order) if batch.pops is not None else None)
@@ -129,4 +149,4 @@ def tune(batch: Batch, mode: str = "refined", trial_steps: int = 100, warmup_ste
     else:
         _, blk, ncg, frc = best
     _cache[key] = (blk, ncg, frc)
-    return blk, SolverConfig(mode=mode, n_cg=ncg, cg_forcing=frc)
+    return blk, dataclasses.replace(base, n_cg=ncg, cg_forcing=frc)

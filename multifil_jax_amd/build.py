@@ -25,14 +25,27 @@ def hipcc() -> str:
 
 
 def build_engine(verbose: bool = False) -> str:
+    """Compile the library from source (always: no up-to-date check), and
+    print one line saying that it did, with the time taken and a digest of
+    the sources."""
+    import hashlib
+    import time
     tmp = OUT + ".tmp"
     cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-8000:])
     if verbose and r.stderr:
         print(r.stderr[-4000:])
     os.replace(tmp, OUT)
+    h = hashlib.sha256()
+    srcs = [os.path.join(_HERE, "csrc", f) for f in sorted(os.listdir(os.path.join(_HERE, "csrc")))]
+    for f in srcs + [os.path.join(ROOT, "include", "multifil_amd.h")]:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    print(f"build: compiled {os.path.relpath(OUT, ROOT)} from source in {time.time() - t0:.1f} s "
+          f"(gfx950, sources sha256 {h.hexdigest()[:16]})", flush=True)
     return OUT
 
 

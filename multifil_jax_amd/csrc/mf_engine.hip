@@ -1410,7 +1410,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   if (L.launch_flags & LAUNCH_TABLES_IN_LDS) {
     LDS float* tl = (LDS float*)(smem + lds_extra);
     const int nt = dm.kxb * (2 * nb * 48 + 2 * nb * 4);
-    for (int i = tid; i < nt; i += BS) tl[i] = tab[TAB_XB + i];
+    // the XB rows follow the instance's ktm TM row sets
+    const float* src = tab + tab_xb(dm.ktm);
+    for (int i = tid; i < nt; i += BS) tl[i] = src[i];
     xbtab = (float*)tl;
     abstab = (float*)(tl + dm.kxb * 2 * nb * 48);
     lds_extra += 4LL * nt;
@@ -1535,13 +1537,13 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       // nearest sites, and the candidate table rows of both permissiveness
       // levels fetched ahead of the barrier (K2 only picks one)
       int q_near[MX], q_bin[MX], q_old[MX];
-      bool q_occ[MX];
+      bool q_occ[MX], q_clr[MX];
       float4 q_r0[MX], q_r1[MX], q_ab[MX];
       float2 q_s0[MX], q_s1[MX];
 #pragma unroll
       for (int q = 0; q < MX; ++q) {
         const int j = tid + q * BS;
-        q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_occ[q] = false;
+        q_near[q] = 0; q_bin[q] = 0; q_old[q] = 0; q_occ[q] = false; q_clr[q] = false;
         q_r0[q] = q_r1[q] = q_ab[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         q_s0[q] = q_s1[q] = make_float2(0.f, 0.f);
         if (j < X) {
@@ -1562,6 +1564,15 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           if (DYN) { q_kold[q] = keyold; q_sold[q] = old; }
           q_bin[q] = lb + bn;
           q_occ[q] = sh.tbound[q_near[q]] >= 0;
+          // The owner of this head's own bound site before any K2 write.
+          // The reference clears every unbinding site before it sets the
+          // new bindings (transitions.py:1203-1219).  A head binds a site
+          // only if the site's owner read -1 here, and an unbinding head's
+          // clear then finds -1 already: so a clear is stored only when
+          // this value is >= 0, no bind can target that site this step, and
+          // a store and an atomicMin never meet on one site.
+          const int own = sh.xsite[j];
+          q_clr[q] = own >= 0 && sh.tbound[own - dm.nk] >= 0;
           q_old[q] = old;
         }
       }
@@ -1608,7 +1619,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const bool can_bind = binding && perm && !q_occ[q];
           int site = sh.xsite[j];
           if (unbinding) {
-            if (site >= 0) sh.tbound[site - dm.nk] = -1;
+            if (site >= 0 && q_clr[q]) sh.tbound[site - dm.nk] = -1;
             site = -1;
           }
           if (can_bind) {

@@ -23,7 +23,10 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .params import N_PARAM_FIELDS
+from .params import N_PARAM_FIELDS, PARAM_INDEX
+
+# driver columns and the parameter-row fields a NaN entry falls back to
+DRIVER_FIELDS = ("z_line", "pCa", "lattice_spacing")
 
 _STATE_FIELDS = ("pos", "ls", "xb_bound", "xb_key", "thin_bound", "xb_state", "tm")
 
@@ -147,6 +150,22 @@ def _torch():
     return torch
 
 
+def resolve_drivers(drivers: np.ndarray, params: np.ndarray) -> np.ndarray:
+    """NaN driver entries take the instance's parameter-row value (the
+    reference's sentinel: core/state.py:331-343 resolve_value, applied per
+    step by timestep.py:97-102 and core/params.py:825-854 with_drivers).
+
+    drivers: (B, T, 3) float32 (z_line, pCa, lattice_spacing); params:
+    (B, 44) float32 rows.  Returns the array itself when it has no NaN, else
+    a resolved copy.
+    """
+    nan = np.isnan(drivers)
+    if not nan.any():
+        return drivers
+    const = params[:, [PARAM_INDEX[n] for n in DRIVER_FIELDS]].astype(np.float32)
+    return np.where(nan, const[:, None, :], drivers).astype(np.float32)
+
+
 def default_block(topos) -> int:
     """Workgroup size for a set of lattices: one wave per filament chain when
     the chains fit in 12 waves (768 threads, 168 VGPRs per lane), else 1024."""
@@ -248,6 +267,7 @@ class Batch:
         if drivers.ndim != 3 or drivers.shape[0] != self.B or drivers.shape[2] != 3:
             raise ValueError(f"drivers must be ({self.B}, T, 3), got {drivers.shape}")
         self.T = drivers.shape[1]
+        drivers = resolve_drivers(drivers, params)
         self.params_np = params
         self.drivers_np = drivers
         self.seeds = np.asarray(seeds, dtype=np.uint64)

@@ -132,3 +132,34 @@ def test_uneven_lengths_and_mixed_lattices(torch_cuda, skeletal):
         assert np.allclose(fm[b, :L], one.force()[0], rtol=1e-9, atol=0), b
         assert np.array_equal(mm[b, :L, 5:12], one.metrics()[0, :, 5:12]), b
         assert np.all(mm[b, L:] == 0.0) and np.all(fm[b, L:] == 0.0), b
+
+
+@pytest.mark.parametrize("lat", [3, 4])
+def test_dynamic_free_run_refined_3x3_4x4(torch_cuda, skeletal, pdict, lat):
+    """The bars of the 2x2 free run at the uneven batch's middle lattices:
+    3x3 (odd rows, five node slots per lane) and 4x4, one instance at a
+    constant Z-line and one on a 10 nm ramp with nu = 0.5."""
+    from multifil_jax_amd.topology import SarcTopology
+    s, d = skeletal
+    topo = SarcTopology.create(lat, lat, s, d)
+    T = 24
+    K = K_LAT * topo.n_thick
+    drv = drivers_const(2, T, np.array([1000.0, 1000.0], np.float32), np.array([4.5, 5.2], np.float32))
+    drv[1, :, 0] = np.linspace(1000.0, 1010.0, T, dtype=np.float32)
+    nu = np.array([0.0, 0.5], np.float32)
+    print(f"dynamic free run {lat}x{lat}", flush=True)
+    bt = _batch([topo], [0, 0], d, drv, [21, 22], dynamic=(np.full(2, K, np.float32), nu))
+    bt.run(T, cfg=_cfg("refined"))
+    fg, mg = bt.force(), bt.metrics()
+    for b in range(2):
+        _, out = mo.run(topo, pdict, drv[b, :, 0], drv[b, :, 1], drv[b, :, 2], 1.0, 21 + b,
+                        mo.SolverConfig(mode="exact"), K_lat=K, nu=float(nu[b]))
+        same = np.all(mg[b, :, 6:12] == out[:, 6:12], axis=1)
+        assert same[:6].all(), f"instance {b}: state counts diverge early"
+        assert np.max(np.abs(fg[b] - out[:, 0])[same] / np.abs(out[same, 0])) < 1e-5
+        assert np.max(np.abs(mg[b, same, 4] - out[same, 4])) < 1e-6 * 14 + 2e-6
+        for col in (48, 49):
+            assert np.all(np.abs(mg[b, same, col] - out[same, col])
+                          <= 2e-5 * np.abs(out[same, col]) + 1e-5), col
+        assert np.all(mg[b, :, 1] <= 1e-5)
+        assert out[-1, 5] > 0

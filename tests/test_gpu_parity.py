@@ -218,3 +218,34 @@ def test_full_size_invariants(torch_cuda, topo22, skeletal):
     # force trace: activated instances generate force above the resting level
     hi = drv[:, 0, 1] < 5.0
     assert np.mean(m[hi, -1, 0] - m[hi, 0, 0]) > 50.0
+
+
+@pytest.mark.parametrize("mode", ["refined", "reference"])
+def test_contested_site_clear_then_bind(torch_cuda, topo22, skeletal, pdict, mode):
+    """Head C unbinds from a site whose owner record already reads free
+    while head D (another wave) binds it in the same step: the reference
+    clears before it binds (transitions.py:1203-1219), so D owns the site.
+    The device step from the same state and draws equals the oracle's."""
+    from gpu_helpers import contested_site_state
+    _, d = skeletal
+    states, tab = oracle_trajectory(topo22, pdict, 1000.0, 4.5, 6, seed=11)
+    st, u_tm, u_xb, C, D, th, site = contested_site_state(topo22, pdict, tab, states[6])
+    cfg = mo.SolverConfig(mode="exact" if mode == "refined" else "reference")
+    new, _ = mo.step(st, topo22, pdict, tab, 1000.0, 4.5, 14.0, 0.0, u_tm, u_xb, cfg,
+                     mo.preconditioner(topo22, pdict))
+    assert new.xb_bound[D] == site and new.thin_bound[th, site] >= 0
+    S = topo22.n_thin * topo22.n_sites
+    B = 4           # the same start four times: four workgroups race independently
+    bt = make_batch([topo22], [0] * B, d.rows(B), drivers_const(B, 1, 1000.0, 4.5), np.arange(B, dtype=np.uint64))
+    U = np.zeros((B, 1, S + topo22.total_xbs), np.float32)
+    for b in range(B):
+        bt.set_state(b, oracle_state_to_device(st, topo22))
+        U[b, 0, :S], U[b, 0, S:] = u_tm, u_xb
+    print(f"contested site: C={C} D={D} thin={th} site={site} mode={mode}", flush=True)
+    bt.run(1, cfg=_cfg(mode), uniforms=torch_cuda.from_numpy(U).cuda())
+    for b in range(B):
+        g = bt.get_state(b)
+        assert np.array_equal(g["xb_state"], new.xb_state)
+        assert np.array_equal(g["xb_bound"], new.xb_bound)
+        assert np.array_equal(g["thin_bound"], new.thin_bound.ravel())
+        assert np.array_equal(g["tm"], new.tm.ravel())

@@ -64,18 +64,27 @@ def test_explicit_all_mutant_is_the_global_scale(torch_cuda, topo22, skeletal):
     _same(glob, _run(topo22, s, dynamic_params=d, subpopulation=Subpopulation.random(1.0, seed=4, **SCALES)))
 
 
-@pytest.mark.parametrize("mode", ["mean_field", "random", "c_zone"])
-def test_subpopulation_parity(torch_cuda, topo22, skeletal, pdict, mode):
+# Scale sets: both kinetics (the XB tables do not fit LDS beside a second
+# XB set at 2x2), only TM (one XB set, copied into LDS after the two TM
+# sets) and only XB.
+SCALE_SETS = {"both": SCALES, "tm_only": dict(tm_k_01=0.5), "xb_only": dict(xb_r01_coeff=3.0)}
+
+
+@pytest.mark.parametrize("mode,scales", [("mean_field", "both"), ("random", "both"), ("c_zone", "both"),
+                                         ("random", "tm_only"), ("c_zone", "tm_only"),
+                                         ("random", "xb_only")])
+def test_subpopulation_parity(torch_cuda, topo22, skeletal, pdict, mode, scales):
     from multifil_jax_amd import Subpopulation
     from multifil_jax_amd.simulation import plan_sweep
     from multifil_jax_amd.subpopulation import population_params
     from multifil_jax_amd.params import DYNAMIC_FIELDS
     s, d = skeletal
-    sp = {"mean_field": lambda: Subpopulation.mean_field(0.4, **SCALES),
-          "random": lambda: Subpopulation.random(0.5, seed=9, **SCALES),
-          "c_zone": lambda: Subpopulation.c_zone(topo22, **SCALES)}[mode]()
+    sc = SCALE_SETS[scales]
+    sp = {"mean_field": lambda: Subpopulation.mean_field(0.4, **sc),
+          "random": lambda: Subpopulation.random(0.5, seed=9, **sc),
+          "c_zone": lambda: Subpopulation.c_zone(topo22, **sc)}[mode]()
     T = 12
-    print(f"subpopulation parity {mode}", flush=True)
+    print(f"subpopulation parity {mode} {scales}", flush=True)
     from multifil_jax_amd import run
     res = run(topo22, duration_ms=T, pCa=4.5, z_line=950.0, rng_seed=3, solver="refined",
               static_params=s, dynamic_params=d, subpopulation=sp)
@@ -87,7 +96,8 @@ def test_subpopulation_parity(torch_cuda, topo22, skeletal, pdict, mode):
                  xb_label=None if rp.xb_mask is None else rp.xb_mask[0],
                  tm_label=None if rp.tm_mask is None else rp.tm_mask[0])
     if mode != "mean_field":
-        assert 0 < rp.xb_mask[0].sum() < rp.xb_mask[0].size
+        mask = rp.xb_mask[0] if rp.has_xb else rp.tm_mask[0]
+        assert 0 < mask.sum() < mask.size
     seed = (3 << 32) | 0
     drv = plan.drivers[0]
     _, out = mo.run(topo22, pdict, drv[:, 0], drv[:, 1], drv[:, 2], 1.0, seed,

@@ -110,3 +110,134 @@ def test_projected_strong_stiffness(pdict):
 
 def test_kT_value():
     assert np.isclose(mo.kT_pNnm(26.15), 1.3810e-23 * 299.3 * 1e21)
+
+
+# ---------------------------------------------------------------------------
+# XB rate laws (rate_functions.py:318-753, transitions.py:632-804), pinned by
+# invariants that tie each rate to another rate or to the force kernel's
+# expression (forces.py:398-399, itself pinned by autograd and the zero
+# points above): E_weak(x) - E_weak(x0) = int f_weak dx, and likewise for
+# the strong configuration.  A dropped term, a flipped sign or a wrong
+# factor in any one rate law breaks the identity it enters.
+# ---------------------------------------------------------------------------
+D0 = 14.0
+XS = np.linspace(-6.0, 30.0, 37)
+
+
+def _gen(p, x, d=D0, perm=1.0, pCa=4.5):
+    Q = mo.xb_generators(p, pCa, d, np.asarray(x, float), perm)
+    return {k: Q[:, i, j] for k, (i, j) in
+            dict(r01=(0, 1), r10=(1, 0), r12=(1, 2), r21=(2, 1), r23=(2, 3), r32=(3, 2),
+                 r34=(3, 4), r40=(4, 0), r04=(0, 4), r05=(0, 5), r50=(5, 0)).items()}
+
+
+def _energy_change(p, x0, x, strong, d=D0):
+    """(E(x) - E(x0)) / kT of one head configuration, from the axial force."""
+    from scipy.integrate import quad
+    kt = mo.kT_pNnm(p['temp_celsius'])
+    return quad(lambda s: _xb_axial_force(p, s, d, strong), x0, x, epsabs=1e-12, epsrel=1e-12)[0] / kt
+
+
+def test_r01_boltzmann_window(pdict):
+    # r01 = perm A01 exp(-E_weak): log r01 changes by -dE_weak / kT
+    g = _gen(pdict, XS)
+    x0 = 10.0
+    g0 = _gen(pdict, [x0])
+    for x, r in zip(XS, g['r01']):
+        assert np.isclose(np.log(r / g0['r01'][0]), -_energy_change(pdict, x0, x, False), rtol=1e-7, atol=1e-9)
+    # the window peaks at the weak zero-force point (14.126 nm at d = 14)
+    from scipy.optimize import minimize_scalar
+    m = minimize_scalar(lambda x: -_gen(pdict, [x])['r01'][0], bounds=(5, 25), method='bounded',
+                        options={'xatol': 1e-6})
+    assert abs(m.x - 14.126) < 1e-3
+    assert np.all(g['r01'] <= pdict['xb_r01_coeff'])
+    assert np.all(_gen(pdict, XS, perm=0.0)['r01'] == 0.0)
+
+
+def test_r10_detailed_balance(pdict):
+    # r10 = (r01 + 0.005) e^(U_loose - U_DRX), U_loose = U_loose0 + E_weak, so
+    # r10 / (r01 + 0.005) * r01 / A01 = e^(U_loose0 - U_DRX0) wherever r10 is
+    # below its cap
+    g = _gen(pdict, XS)
+    ok = g['r10'] < 1e4
+    assert ok.sum() > 10
+    lhs = g['r10'][ok] / (g['r01'][ok] + 0.005) * g['r01'][ok] / pdict['xb_r01_coeff']
+    assert np.allclose(lhs, np.exp(pdict['xb_U_loose'] - pdict['xb_U_DRX']), rtol=1e-9)
+    # a covered site: r01 = 0, r10 = 0.005 e^(U_loose - U_DRX) still strain dependent
+    g0 = _gen(pdict, XS, perm=0.0)
+    kt = mo.kT_pNnm(pdict['temp_celsius'])
+    ok0 = g0['r10'] < 1e4
+    e = np.array([_energy_change(pdict, 10.0, x, False) for x in XS[ok0]])
+    ref = 0.005 * np.exp(pdict['xb_U_loose'] - pdict['xb_U_DRX'])
+    ew10 = -np.log(_gen(pdict, [10.0])['r01'][0] / pdict['xb_r01_coeff'])
+    assert np.allclose(g0['r10'][ok0], ref * np.exp(ew10 + e), rtol=1e-7)
+    assert kt > 0
+
+
+def test_r12_half_energy_difference(pdict):
+    # r12 = A12 e^(min(E_diff, 30) / 2), E_diff = E_weak - E_strong: log r12
+    # changes by (dE_weak - dE_strong) / 2 kT, and r12 = A12 where E_diff = 0
+    g = _gen(pdict, XS)
+    x0 = 8.0
+    g0 = _gen(pdict, [x0])
+    for x, r in zip(XS, g['r12']):
+        de = _energy_change(pdict, x0, x, False) - _energy_change(pdict, x0, x, True)
+        ediff0 = 2.0 * np.log(g0['r12'][0] / pdict['xb_r12_coeff'])
+        if ediff0 + de < 30.0:
+            assert np.isclose(np.log(r / g0['r12'][0]), 0.5 * de, rtol=1e-7, atol=1e-9), x
+        else:
+            assert np.isclose(r, pdict['xb_r12_coeff'] * np.exp(15.0), rtol=1e-9)
+
+
+def test_r21_detailed_balance(pdict):
+    # r21 = r12 e^(U_t1 - U_loose) = r12 e^(U_t1_0 - U_loose0 - E_diff) and
+    # r12 / A12 = e^(E_diff / 2): (r21 / r12) (r12 / A12)^2 = e^(U_t1_0 - U_loose0)
+    g = _gen(pdict, XS)
+    ediff = 2.0 * np.log(g['r12'] / pdict['xb_r12_coeff'])
+    ok = (g['r21'] < 1e4) & (ediff < 30.0 - 1e-9)
+    assert ok.sum() > 10
+    lhs = g['r21'][ok] / g['r12'][ok] * (g['r12'][ok] / pdict['xb_r12_coeff']) ** 2
+    assert np.allclose(lhs, np.exp(pdict['xb_U_tight_1'] - pdict['xb_U_loose']), rtol=1e-9)
+
+
+def _f_strong(p, x, d=D0):
+    # the rates' strong-state force, transitions.py:737 (not the force
+    # kernel's expression): g_k (r - g_rest) + (1/r) c_k (theta - c_rest)
+    r = np.hypot(x, d)
+    th = np.arctan2(d, x)
+    return p['xb_g_k_strong'] * (r - p['xb_g_rest_strong']) + p['xb_c_k_strong'] * (th - p['xb_c_rest_strong']) / r
+
+
+def test_bell_and_slip_laws(pdict):
+    # r23 = A23 e^(-f d23 / kT) (resisting load slows it), r34 = A34 e^(+f d34 / kT)
+    kt = mo.kT_pNnm(pdict['temp_celsius'])
+    g = _gen(pdict, XS)
+    f = _f_strong(pdict, XS)
+    ok = g['r23'] < 1e4
+    assert np.allclose(g['r23'][ok] * np.exp(f[ok] * pdict['xb_delta_23'] / kt), pdict['xb_r23_coeff'], rtol=1e-9)
+    ok = g['r34'] < 1e4
+    assert ok.sum() > 10
+    assert np.allclose(g['r34'][ok] * np.exp(-f[ok] * pdict['xb_delta_34'] / kt), pdict['xb_r34_coeff'], rtol=1e-9)
+    # monotone in the load: more force, slower 2->3, faster 3->4
+    order = np.argsort(f)
+    assert np.all(np.diff(g['r23'][order]) <= 0) and np.all(np.diff(g['r34'][order]) >= 0)
+    assert (g['r34'] == 1e4).any()          # the cap binds at large strain
+
+
+def test_r32_spans_the_stated_range(pdict):
+    # rate_functions.py:541-542: over x in [-4, 21.5] nm r32 varies ~8.5e5-fold
+    r32 = _gen(pdict, np.linspace(-4.0, 21.5, 4001))['r32']
+    assert 8.0e5 < r32.max() / r32.min() < 9.0e5
+
+
+def test_state_independent_rates(pdict):
+    # r40, r04, r05 constants; r50 a Hill function of [Ca] (rate_functions.py:726)
+    for pCa in (9.0, 6.0, 4.5):
+        g = _gen(pdict, XS, pCa=pCa)
+        for k in ('r40', 'r04', 'r05', 'r50'):
+            assert np.all(g[k] == g[k][0])
+        ca, b = 10.0 ** -pCa, pdict['xb_srx_b']
+        hill = pdict['xb_srx_k0'] + (pdict['xb_srx_kmax'] - pdict['xb_srx_k0']) * ca ** b / (
+            pdict['xb_srx_ca50'] ** b + ca ** b)
+        assert np.isclose(g['r50'][0], hill, rtol=1e-12)
+        assert g['r40'][0] == pdict['xb_r40'] and g['r04'][0] == pdict['xb_r04']
