@@ -190,6 +190,24 @@ int mf_workspace_bytes(const mf_launch* launch, int64_t* per_instance);
 /* Dynamic LDS bytes and the workgroup size mf_run would use (diagnostics). */
 int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes);
 
+/* Size of the library's static device scratch array (per GPU, held until the
+ * process ends). */
+#define MF_NOALLOC_SCRATCH_BYTES (64LL << 20)
+
+/* mf_run without a caller workspace, for callers that cannot allocate device
+ * memory (code under stream / graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference, which leaves all
+ * buffers to XLA.  L->workspace and L->workspace_stride are ignored: the
+ * kernels that keep per-instance arrays in device memory (lattices too large
+ * for LDS) use a file-scope __device__ array of MF_NOALLOC_SCRATCH_BYTES
+ * instead, B * mf_workspace_bytes (rounded up to 256) of it; a larger launch
+ * is rejected with an error.  The call itself allocates nothing (no
+ * hipMalloc, no host allocation); its device address is looked up once per
+ * device on the first call.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two streams, two threads) race.  Nothing in
+ * it has to start at zero.  Results are identical to mf_run's. */
+int mf_run_noalloc(const mf_launch* L, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,10 @@ MF_POP_XB = 1
 MF_POP_TM = 2
 
 EXPORTED = ("mf_abi_version", "mf_last_error", "mf_state_bytes", "mf_table_floats",
-            "mf_table_floats_pop", "mf_tables", "mf_run", "mf_run_lds_bytes", "mf_workspace_bytes")
+            "mf_table_floats_pop", "mf_tables", "mf_run", "mf_run_lds_bytes", "mf_workspace_bytes",
+            "mf_run_noalloc")
+# bytes of the library's static scratch array (mf_run_noalloc)
+MF_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class MfLattice(C.Structure):
@@ -95,7 +98,7 @@ def load() -> C.CDLL:
     lib.mf_table_floats.argtypes = [C.c_int32]
     lib.mf_table_floats_pop.restype = C.c_int64
     lib.mf_table_floats_pop.argtypes = [C.c_int32] * 3
-    for fn in ("mf_tables", "mf_run"):
+    for fn in ("mf_tables", "mf_run", "mf_run_noalloc"):
         getattr(lib, fn).restype = C.c_int
         getattr(lib, fn).argtypes = [C.POINTER(MfLaunch), C.c_void_p]
     for fn in ("mf_run_lds_bytes", "mf_workspace_bytes"):

@@ -11,12 +11,12 @@
 #include "mf_common.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <string>
 #include <type_traits>
-#include <vector>
 
 using namespace mf;
 
@@ -35,12 +35,20 @@ constexpr int LAUNCH_FACE_POS = 1 << 17;
 
 namespace {
 
-thread_local std::string g_err;
-int fail(const std::string& m) { g_err = m; return 1; }
-#define HIPCHK(x)                                                                   \
-  do {                                                                              \
-    hipError_t e_ = (x);                                                            \
-    if (e_ != hipSuccess) return fail(std::string(#x ": ") + hipGetErrorString(e_)); \
+// Error text of the last failed call on this thread, in a fixed buffer:
+// reporting an error allocates nothing.
+thread_local char g_err[512];
+__attribute__((format(printf, 1, 2))) int fail(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return 1;
+}
+#define HIPCHK(x)                                                     \
+  do {                                                                \
+    hipError_t e_ = (x);                                              \
+    if (e_ != hipSuccess) return fail("%s: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
 // ---------------------------------------------------------------------------
@@ -1978,7 +1986,7 @@ void needs(const Dims& d, int bs, int& ms, int& mx) {
 // First variant (LDS-resident ones come first) that fits every lattice of
 // the launch; *lds its dynamic LDS bytes, *ws its workspace bytes per
 // instance (0 unless it is a global-workspace variant).
-int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant** out, int64_t* lds,
+int pick(const mf_launch* L, const mf_lattice* lats, int n_lats, const Variant** out, int64_t* lds,
          int64_t* ws) {
   int n;
   const Variant* v = variants(n);
@@ -1988,8 +1996,8 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
     if (v[i].ref64 != r64 || v[i].dyn != dyn || v[i].bs != L->block) continue;
     bool ok = true;
     int64_t need = 0, wneed = 0;
-    for (const auto& G : lats) {
-      const Dims d = dims_of(G, *L);
+    for (int g = 0; g < n_lats; ++g) {
+      const Dims d = dims_of(lats[g], *L);
       int ms, mx;
       needs(d, v[i].bs, ms, mx);
       int nchw = (d.nch + v[i].bs / WAVE - 1) / (v[i].bs / WAVE);
@@ -2008,19 +2016,21 @@ int pick(const mf_launch* L, const std::vector<mf_lattice>& lats, const Variant*
       return 0;
     }
   }
-  return fail("no kernel variant fits this lattice at block " + std::to_string(L->block) +
-              " (LDS or register slots exceeded)");
+  return fail("no kernel variant fits this lattice at block %d (LDS or register slots exceeded)", L->block);
 }
 
-int fetch_lattices(const mf_launch* L, std::vector<mf_lattice>& lats) {
+// Copy the launch's lattice descriptors into lats[0, cap) and validate the
+// launch.  With lattices_host set this touches no device memory.
+int fetch_lattices(const mf_launch* L, mf_lattice* lats, int cap) {
   if (L->n_lattices <= 0 || L->B <= 0) return fail("empty launch");
-  lats.resize(L->n_lattices);
+  if (L->n_lattices > cap) return fail("%d lattices in one launch, at most %d", L->n_lattices, cap);
   if (L->lattices_host) {
-    std::memcpy(lats.data(), L->lattices_host, sizeof(mf_lattice) * L->n_lattices);
+    std::memcpy(lats, L->lattices_host, sizeof(mf_lattice) * L->n_lattices);
   } else {
-    HIPCHK(hipMemcpy(lats.data(), L->lattices, sizeof(mf_lattice) * L->n_lattices, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lats, L->lattices, sizeof(mf_lattice) * L->n_lattices, hipMemcpyDeviceToHost));
   }
-  for (const auto& G : lats) {
+  for (int g = 0; g < L->n_lattices; ++g) {
+    const mf_lattice& G = lats[g];
     if (G.n_bins <= 0 || G.n_bins > MF_MAX_BINS) return fail("n_bins out of range");
     if (G.n_xb_per_crown <= 0 || G.width <= 0) return fail("bad lattice");
   }
@@ -2036,12 +2046,86 @@ int fetch_lattices(const mf_launch* L, std::vector<mf_lattice>& lats) {
   return 0;
 }
 
+// Lattices one launch may carry (descriptors are copied to a host array of
+// this size, on the stack).
+constexpr int MAX_LATTICES = 64;
+
+int check_run_args(const mf_launch* L) {
+  if (!L) return fail("null launch");
+  if (L->block != 256 && L->block != 512 && L->block != 768 && L->block != 1024)
+    return fail("block must be 256, 512, 768 or 1024");
+  if (L->cfg.n_cg < 1 || L->cfg.n_cg > 64) return fail("n_cg out of range");
+  return 0;
+}
+
+// Launch the step kernel v for L, with the per-instance workspace at ws
+// (stride ws_stride bytes; unused when the variant keeps everything in LDS).
+int launch_run(const mf_launch* L, const mf_lattice* lats, const Variant* v, int64_t lds, uint8_t* ws,
+               int64_t ws_stride, void* stream) {
+  // transition rows into LDS when they fit beside the instance arrays
+  int64_t tab_bytes = 0;
+  const int kxb = table_sets(*L, MF_POP_XB);
+  for (int g = 0; g < L->n_lattices; ++g)
+    tab_bytes = std::max<int64_t>(tab_bytes, 4LL * kxb * (2LL * lats[g].n_bins * 48 + 2LL * lats[g].n_bins * 4));
+  mf_launch Lk = *L;
+  Lk.workspace = ws;
+  Lk.workspace_stride = ws_stride;
+  Lk.launch_flags &= ~(LAUNCH_TABLES_IN_LDS | LAUNCH_FACE_POS);
+  if (lds + tab_bytes <= 160 * 1024) {
+    Lk.launch_flags |= LAUNCH_TABLES_IN_LDS;
+    lds += tab_bytes;
+  }
+  // then the candidate-site positions of the nearest-site search
+  int64_t fpos_bytes = 0;
+  for (int g = 0; g < L->n_lattices; ++g)
+    fpos_bytes = std::max<int64_t>(fpos_bytes, 4LL * lats[g].n_thin * lats[g].n_faces * ((lats[g].width + 3) & ~3));
+  fpos_bytes = 16 + ((fpos_bytes + 15) & ~int64_t(15));  // + alignment slack of the kernel's offset
+  if (lds + fpos_bytes <= 160 * 1024) {
+    Lk.launch_flags |= LAUNCH_FACE_POS;
+    lds += fpos_bytes;
+  }
+  HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  void* args[] = {(void*)&Lk};
+  HIPCHK(hipLaunchKernel(v->fn, dim3(L->B), dim3(v->bs), args, (size_t)lds, (hipStream_t)stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free entry point: the workspace of the global-workspace
+// variants lives in this file-scope device array.  Every GPU holds its own
+// copy until the process ends; a launch uses its first B * stride bytes.
+// ---------------------------------------------------------------------------
+__device__ __align__(256) uint8_t scratch[MF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::atomic<uint8_t*> g_scratch[MAX_DEVICES];
+
+// The device address of `scratch` on the current device, looked up on the
+// first call made with that device current (the host-side name is only a
+// stand-in and must never reach a kernel).
+int scratch_ptr(uint8_t** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail("device %d beyond the %d the library tracks", dev, MAX_DEVICES);
+  uint8_t* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    void* q = nullptr;
+    HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    if (!q) return fail("hipGetSymbolAddress returned no address for the scratch array");
+    p = (uint8_t*)q;
+    g_scratch[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 int mf_abi_version(void) { return MF_ABI_VERSION; }
-const char* mf_last_error(void) { return g_err.c_str(); }
+const char* mf_last_error(void) { return g_err; }
 
 int64_t mf_state_bytes(int32_t n_thick, int32_t n_crowns, int32_t n_thin, int32_t n_sites,
                        int32_t n_xb_per_crown) {
@@ -2055,10 +2139,10 @@ int64_t mf_table_floats_pop(int32_t n_bins, int32_t n_pop_tm, int32_t n_pop_xb) 
 
 int mf_tables(const mf_launch* L, void* stream) {
   if (!L) return fail("null launch");
-  std::vector<mf_lattice> lats;
-  if (int e = fetch_lattices(L, lats)) return e;
+  mf_lattice lats[MAX_LATTICES];
+  if (int e = fetch_lattices(L, lats, MAX_LATTICES)) return e;
   int nb_max = 0;
-  for (const auto& G : lats) nb_max = std::max(nb_max, G.n_bins);
+  for (int g = 0; g < L->n_lattices; ++g) nb_max = std::max(nb_max, lats[g].n_bins);
   const int64_t total =
       (int64_t)L->B * (table_sets(*L, MF_POP_TM) * 27 + 4 * nb_max * table_sets(*L, MF_POP_XB));
   const int grid = (int)((total + 255) / 256);
@@ -2069,59 +2153,49 @@ int mf_tables(const mf_launch* L, void* stream) {
 
 int mf_run_lds_bytes(const mf_launch* L, int64_t* lds_bytes) {
   if (!L || !lds_bytes) return fail("null argument");
-  std::vector<mf_lattice> lats;
-  if (int e = fetch_lattices(L, lats)) return e;
+  mf_lattice lats[MAX_LATTICES];
+  if (int e = fetch_lattices(L, lats, MAX_LATTICES)) return e;
   const Variant* v;
-  return pick(L, lats, &v, lds_bytes, nullptr);
+  return pick(L, lats, L->n_lattices, &v, lds_bytes, nullptr);
 }
 
 int mf_workspace_bytes(const mf_launch* L, int64_t* per_instance) {
   if (!L || !per_instance) return fail("null argument");
-  std::vector<mf_lattice> lats;
-  if (int e = fetch_lattices(L, lats)) return e;
+  mf_lattice lats[MAX_LATTICES];
+  if (int e = fetch_lattices(L, lats, MAX_LATTICES)) return e;
   const Variant* v;
   int64_t lds;
-  return pick(L, lats, &v, &lds, per_instance);
+  return pick(L, lats, L->n_lattices, &v, &lds, per_instance);
 }
 
 int mf_run(const mf_launch* L, void* stream) {
-  if (!L) return fail("null launch");
-  if (L->block != 256 && L->block != 512 && L->block != 768 && L->block != 1024)
-    return fail("block must be 256, 512, 768 or 1024");
-  if (L->cfg.n_cg < 1 || L->cfg.n_cg > 64) return fail("n_cg out of range");
-  std::vector<mf_lattice> lats;
-  if (int e = fetch_lattices(L, lats)) return e;
+  if (int e = check_run_args(L)) return e;
+  mf_lattice lats[MAX_LATTICES];
+  if (int e = fetch_lattices(L, lats, MAX_LATTICES)) return e;
   const Variant* v;
   int64_t lds, ws = 0;
-  if (int e = pick(L, lats, &v, &lds, &ws)) return e;
+  if (int e = pick(L, lats, L->n_lattices, &v, &lds, &ws)) return e;
   if (ws > 0 && (L->workspace == nullptr || L->workspace_stride < ws))
-    return fail("this launch needs a device workspace of " + std::to_string(ws) +
-                " bytes per instance (mf_workspace_bytes)");
-  // transition rows into LDS when they fit beside the instance arrays
-  int64_t tab_bytes = 0;
-  const int kxb = table_sets(*L, MF_POP_XB);
-  for (const auto& G : lats)
-    tab_bytes = std::max<int64_t>(tab_bytes, 4LL * kxb * (2LL * G.n_bins * 48 + 2LL * G.n_bins * 4));
-  mf_launch Lk = *L;
-  Lk.launch_flags &= ~(LAUNCH_TABLES_IN_LDS | LAUNCH_FACE_POS);
-  if (lds + tab_bytes <= 160 * 1024) {
-    Lk.launch_flags |= LAUNCH_TABLES_IN_LDS;
-    lds += tab_bytes;
-  }
-  // then the candidate-site positions of the nearest-site search
-  int64_t fpos_bytes = 0;
-  for (const auto& G : lats)
-    fpos_bytes = std::max<int64_t>(fpos_bytes, 4LL * G.n_thin * G.n_faces * ((G.width + 3) & ~3));
-  fpos_bytes = 16 + ((fpos_bytes + 15) & ~int64_t(15));  // + alignment slack of the kernel's offset
-  if (lds + fpos_bytes <= 160 * 1024) {
-    Lk.launch_flags |= LAUNCH_FACE_POS;
-    lds += fpos_bytes;
-  }
-  HIPCHK(hipFuncSetAttribute(v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  void* args[] = {(void*)&Lk};
-  HIPCHK(hipLaunchKernel(v->fn, dim3(L->B), dim3(v->bs), args, (size_t)lds, (hipStream_t)stream));
-  HIPCHK(hipGetLastError());
-  return 0;
+    return fail("this launch needs a device workspace of %lld bytes per instance (mf_workspace_bytes)",
+                (long long)ws);
+  return launch_run(L, lats, v, lds, L->workspace, L->workspace_stride, stream);
+}
+
+int mf_run_noalloc(const mf_launch* L, void* stream) {
+  if (int e = check_run_args(L)) return e;
+  mf_lattice lats[MAX_LATTICES];
+  if (int e = fetch_lattices(L, lats, MAX_LATTICES)) return e;
+  const Variant* v;
+  int64_t lds, ws = 0;
+  if (int e = pick(L, lats, L->n_lattices, &v, &lds, &ws)) return e;
+  const int64_t stride = (ws + 255) & ~int64_t(255);
+  if (stride > 0 && stride * (int64_t)L->B > MF_NOALLOC_SCRATCH_BYTES)
+    return fail("mf_run_noalloc: %d instances need %lld bytes of scratch, more than the %lld of the static array",
+                L->B, (long long)(stride * (int64_t)L->B), (long long)MF_NOALLOC_SCRATCH_BYTES);
+  uint8_t* p = nullptr;
+  if (stride > 0)
+    if (int e = scratch_ptr(&p)) return e;
+  return launch_run(L, lats, v, lds, p, stride, stream);
 }
 
 }  // extern "C"

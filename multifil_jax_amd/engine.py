@@ -437,7 +437,7 @@ class Batch:
             init = np.zeros_like(init)
 
     def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
-            block: Optional[int] = None, sync: bool = True, kernel_events=None):
+            block: Optional[int] = None, sync: bool = True, kernel_events=None, noalloc: bool = False):
         """Advance every instance from step t0 by n_steps (the first call
         equilibrates each instance before its first step, as the reference's
         create_and_equilibrate does).
@@ -446,6 +446,9 @@ class Batch:
         (TM sites first, then XBs) instead of the device Philox stream.
         kernel_events: optional (start, end) torch.cuda.Event pair recorded on
         the launch stream immediately around the step-kernel launches.
+        noalloc: launch through mf_run_noalloc, whose large-lattice kernels
+        use the library's static device scratch instead of a workspace this
+        batch allocates.
         """
         cfg = cfg or SolverConfig()
         if block is not None:
@@ -474,10 +477,14 @@ class Batch:
                 _lib.check(self.lib.mf_tables(C.byref(L), stream), "mf_tables")
                 self._table_rows = rows.copy()
                 self._table_dt = cfg.dt
-            self._workspace(L)
+            if not noalloc:
+                self._workspace(L)
             if kernel_events is not None and s0 == segs[0][0]:
                 kernel_events[0].record(_torch().cuda.current_stream(self.device))
-            _lib.check(self.lib.mf_run(C.byref(L), stream), "mf_run")
+            if noalloc:
+                _lib.check(self.lib.mf_run_noalloc(C.byref(L), stream), "mf_run_noalloc")
+            else:
+                _lib.check(self.lib.mf_run(C.byref(L), stream), "mf_run")
             if kernel_events is not None and s0 == segs[-1][0]:
                 kernel_events[1].record(_torch().cuda.current_stream(self.device))
             self.initialized[:] = True

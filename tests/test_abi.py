@@ -116,3 +116,26 @@ def test_subpopulation_launch_validation():
     assert lib.mf_table_floats_pop(200, 1, 1) == lib.mf_table_floats(200)
     assert lib.mf_table_floats_pop(200, 2, 3) == 16 + 2 * 27 * 16 + 3 * (400 * 48 + 400 * 4)
     assert lib.mf_table_floats_pop(200, 0, 1) < 0
+
+
+def test_noalloc_rejects_launches_beyond_its_scratch(skeletal):
+    """mf_run_noalloc checks B x workspace against its static array before
+    touching the device: a 6x6 launch of 400 instances (~85 MB of
+    workspace) is refused with an error, 280 fit (host-side check only)."""
+    from multifil_jax_amd import _lib
+    from multifil_jax_amd.engine import LatticePack, SolverConfig
+    from multifil_jax_amd.topology import SarcTopology
+    s, d = skeletal
+    lib = _lib.load()
+    pack = LatticePack([SarcTopology.create(6, 6, s, d)])
+    L = _lib.MfLaunch()
+    L.n_lattices, L.block, L.n_pops = 1, 1024, 1
+    L.cfg = SolverConfig(mode="refined").ctype()
+    L.lattices_host = ctypes.addressof(pack.desc_host)
+    per = ctypes.c_int64(0)
+    L.B = 400
+    assert lib.mf_workspace_bytes(ctypes.byref(L), ctypes.byref(per)) == 0 and per.value > 0
+    stride = (per.value + 255) // 256 * 256
+    assert stride * 400 > _lib.MF_NOALLOC_SCRATCH_BYTES >= stride * 280
+    assert lib.mf_run_noalloc(ctypes.byref(L), None) != 0
+    assert b"scratch" in lib.mf_last_error()
