@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MF_ABI_VERSION 4
+#define MF_ABI_VERSION 5
 #define MF_N_PARAMS 44      /* DynamicParams fields, params.py:73-553 order */
 #define MF_N_METRICS 52     /* metrics_fn.py:343-417 order                */
 #define MF_MAX_BINS 256     /* n_xb_bins upper bound                       */
@@ -63,6 +63,15 @@ extern "C" {
 #define MF_LAUNCH_DYNAMIC 1 /* some instance has MF_INST_DYNAMIC: run the
                                kernels that carry the bordered (N+1) solve
                                of solver.py:599-656                        */
+#define MF_LAUNCH_KINETICS_ONLY 2 /* each step is kinetics_step alone
+                               (timestep.py:65-129): nearest sites, TM and
+                               XB transitions; no solve, no metrics row    */
+#define MF_LAUNCH_SOLVE_ONLY 4 /* each step is solve_equilibrium alone
+                               (solver.py:663-771) at the step's Z-line and
+                               spacing (dynamic: d_ref = spacing, the state's
+                               spacing the first guess); the metrics row
+                               carries force, residual, spacing and
+                               iterations, its kinetic columns are void    */
 
 /* One lattice (SarcTopology.create output, packed).  All offsets are in
  * elements into the float / int32 lattice blobs. */
@@ -160,6 +169,11 @@ typedef struct mf_launch {
  *   f64 spacing   (carried lattice spacing)
  *   i32 xb_bound[X]   (-1 unbound)          X = total_xbs
  *   i32 xb_key[X]     (2*n_bins table row of the last kinetics step)
+ *   i32 xb_near[X]    (nearest site along the target thin filament and
+ *   f32 xb_xdist[X]    its axial offset from the crown, from the launch's
+ *                      last nearest-site update: the reference's
+ *                      xb_nearest_bs and xb_distances[..., 0]; written by
+ *                      the kernel, never read)
  *   i32 thin_bound[S] (-1 unbound)          S = n_thin*n_sites
  *   u8  xb_state[X], u8 tm_state[S]
  * mf_state_bytes returns the size for a lattice. */

@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmultifil_amd.so")
 
-MF_ABI_VERSION = 4
+MF_ABI_VERSION = 5
 MF_N_PARAMS = 44
 MF_N_METRICS = 52
 MF_MAX_BINS = 256
@@ -22,6 +22,8 @@ MF_SOLVER_REFINED = 1
 MF_INST_INIT = 1
 MF_INST_DYNAMIC = 2
 MF_LAUNCH_DYNAMIC = 1
+MF_LAUNCH_KINETICS_ONLY = 2
+MF_LAUNCH_SOLVE_ONLY = 4
 MF_POP_NONE = 0
 MF_POP_MEAN_FIELD = 1
 MF_POP_EXPLICIT = 2

@@ -55,7 +55,7 @@ __attribute__((format(printf, 1, 2))) int fail(const char* fmt, ...) {
 // state layout
 // ---------------------------------------------------------------------------
 struct StateLayout {
-  int64_t pos, ls, xb_bound, xb_key, thin_bound, xb_state, tm, bytes;
+  int64_t pos, ls, xb_bound, xb_key, xb_near, xb_xdist, thin_bound, xb_state, tm, bytes;
 };
 __host__ __device__ inline int64_t al8(int64_t v) { return (v + 7) & ~int64_t(7); }
 __host__ __device__ inline StateLayout state_layout(int Tk, int nc, int Tn, int ns, int nx) {
@@ -66,7 +66,9 @@ __host__ __device__ inline StateLayout state_layout(int Tk, int nc, int Tn, int 
   L.ls = L.pos + 8 * N;
   L.xb_bound = L.ls + 8;
   L.xb_key = al8(L.xb_bound + 4 * X);
-  L.thin_bound = al8(L.xb_key + 4 * X);
+  L.xb_near = al8(L.xb_key + 4 * X);
+  L.xb_xdist = al8(L.xb_near + 4 * X);
+  L.thin_bound = al8(L.xb_xdist + 4 * X);
   L.xb_state = al8(L.thin_bound + 4 * S);
   L.tm = L.xb_state + X;
   L.bytes = al8(L.tm + S);
@@ -1286,6 +1288,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   float* tab = L.tables + I.table_off;
   const StateLayout SL = state_layout(dm.Tk, dm.nc, dm.Tn, dm.ns, dm.nx);
   uint8_t* stb = L.state + I.state_off;
+  int32_t* st_near = (int32_t*)(stb + SL.xb_near);
+  float* st_xdist = (float*)(stb + SL.xb_xdist);
   const int S = dm.S, X = dm.X, nb = dm.nb;
 
   // ---- static per-workgroup data ------------------------------------------
@@ -1440,6 +1444,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   if (n_steps > 0 && t_first > 0) z_prev = drv[(t_first - 1) * 3];
   const int tid_k = tid;
   bool fpos_ready = false;  // fpos already holds this step's candidate positions
+  // the split halves of a step (public launch flags, uniform per launch)
+  const bool kin_only = (L.launch_flags & MF_LAUNCH_KINETICS_ONLY) != 0;
+  const bool solve_only = (L.launch_flags & MF_LAUNCH_SOLVE_ONLY) != 0;
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
     // The thread index is made opaque once per step: otherwise the compiler
     // hoists every tid-derived LDS address of the strided per-head and
@@ -1471,10 +1478,26 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         nearest_site<REF64>(sh, dm, slot, slot / dm.nf, j / dm.nx, nr, xd, fpos, Wp);
         // tm == 0 everywhere: permissiveness 0
         sh.xkey[j] = (short)((xlab ? xlab[j] * 2 * nb : 0) + bin_of(sh.edges, nb, inv_w, xd));
+        if (n_steps == 0) { st_near[j] = nr; st_xdist[j] = xd; }
       }
       __syncthreads();
       z_solve = sh.prm[P_ZLINE];
       d_solve = sh.prm[P_LS];
+    } else if (solve_only) {
+      // solve_equilibrium alone (solver.py:663-771) on the loaded state
+      z_prev = zl;
+      z_solve = zl;
+      d_solve = lsd;
+      if (DYN && rad.on) rad.dref = lsd * powf(z_trace0 / zl, I.nu);
+      // the site -> bound-head lists the solve walks, which K2 builds in a
+      // full step: cleared, then every head with a site inserted
+      for (int s2 = tid; s2 < S; s2 += BS) sh.shead[s2] = -1;
+      __syncthreads();
+      for (int j = tid; j < X; j += BS) {
+        const int site = sh.xsite[j];
+        if (site >= 0) sh.xnext[j] = (short)atomicExch((int*)(sh.shead + (site - dm.nk)), j);
+      }
+      __syncthreads();
     } else {
       const float dz = t > 0 ? zl - z_prev : 0.0f;
       z_prev = zl;
@@ -1561,6 +1584,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const int slot = sh.xslot[j] & 0x7fff;
           const int thin = slot / dm.nf;
           nearest_site<REF64>(sh, dm, slot, thin, j / dm.nx, q_near[q], xd, fpos, Wp);
+          // the launch's last nearest-site update goes to the state record
+          // (xb_nearest_bs, xb_distances[..., 0] of the reference's State)
+          if (step == n_steps - 1) { st_near[j] = q_near[q]; st_xdist[j] = xd; }
           q_near[q] += thin * dm.ns;  // thin-global site index from here on
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
           const int lb = xlab ? xlab[j] * 2 * nb : 0;
@@ -1665,6 +1691,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       }
       __syncthreads();
     }  // kinetics
+    // kinetics_step alone (timestep.py:65-129): no solve, no metrics row
+    if (kin && kin_only) continue;
 
     // ---- equilibrium solve -------------------------------------------------
     int iters;
@@ -1679,7 +1707,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     if (!kin) continue;
     const double ls_new = (DYN && rad.on) ? (double)d_sol : (double)d_solve;
     ls_carry = ls_new;
-    if (DYN && rad.on) {
+    if (DYN && rad.on && !solve_only) {
       // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
       // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
       xb_table_rebuild(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS);

@@ -28,7 +28,7 @@ from .params import N_PARAM_FIELDS, PARAM_INDEX
 # driver columns and the parameter-row fields a NaN entry falls back to
 DRIVER_FIELDS = ("z_line", "pCa", "lattice_spacing")
 
-_STATE_FIELDS = ("pos", "ls", "xb_bound", "xb_key", "thin_bound", "xb_state", "tm")
+_STATE_FIELDS = ("pos", "ls", "xb_bound", "xb_key", "xb_near", "xb_xdist", "thin_bound", "xb_state", "tm")
 
 
 def _al8(v: int) -> int:
@@ -42,7 +42,9 @@ def state_layout(topo) -> dict:
     o["ls"] = 8 * N
     o["xb_bound"] = o["ls"] + 8
     o["xb_key"] = _al8(o["xb_bound"] + 4 * X)
-    o["thin_bound"] = _al8(o["xb_key"] + 4 * X)
+    o["xb_near"] = _al8(o["xb_key"] + 4 * X)
+    o["xb_xdist"] = _al8(o["xb_near"] + 4 * X)
+    o["thin_bound"] = _al8(o["xb_xdist"] + 4 * X)
     o["xb_state"] = _al8(o["thin_bound"] + 4 * S)
     o["tm"] = o["xb_state"] + X
     o["bytes"] = _al8(o["tm"] + S)
@@ -437,7 +439,8 @@ class Batch:
             init = np.zeros_like(init)
 
     def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
-            block: Optional[int] = None, sync: bool = True, kernel_events=None, noalloc: bool = False):
+            block: Optional[int] = None, sync: bool = True, kernel_events=None, noalloc: bool = False,
+            launch_flags: int = 0):
         """Advance every instance from step t0 by n_steps (the first call
         equilibrates each instance before its first step, as the reference's
         create_and_equilibrate does).
@@ -449,6 +452,8 @@ class Batch:
         noalloc: launch through mf_run_noalloc, whose large-lattice kernels
         use the library's static device scratch instead of a workspace this
         batch allocates.
+        launch_flags: extra MF_LAUNCH_* bits (MF_LAUNCH_KINETICS_ONLY,
+        MF_LAUNCH_SOLVE_ONLY: each step runs that half of the step alone).
         """
         cfg = cfg or SolverConfig()
         if block is not None:
@@ -471,6 +476,7 @@ class Batch:
             self._instances(n, init, s0)
             L = self._launch(cfg, uniforms, u_stride)
             L.t_base = int(s0)
+            L.launch_flags |= int(launch_flags)
             rows = self.drivers_np[:, s0, 1:3]
             if self._table_rows is None or not np.array_equal(rows, self._table_rows) \
                     or cfg.dt != self._table_dt:
@@ -491,6 +497,26 @@ class Batch:
         if sync:
             _torch().cuda.synchronize(self.device)
 
+    def update_inputs(self, params=None, drivers=None):
+        """Replace the parameter rows and / or the driver trace in place
+        (same shapes; NaN drivers resolved as at construction).  The next
+        run rebuilds the transition tables."""
+        torch = _torch()
+        if params is not None:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            if params.shape != self.params_np.shape:
+                raise ValueError(f"params must be {self.params_np.shape}, got {params.shape}")
+            self.params_np = params
+            self.d_params.copy_(torch.from_numpy(params))
+        if drivers is not None:
+            drivers = np.ascontiguousarray(drivers, dtype=np.float32)
+            if drivers.shape != self.drivers_np.shape:
+                raise ValueError(f"drivers must be {self.drivers_np.shape}, got {drivers.shape}")
+            self.drivers_np = resolve_drivers(drivers, self.params_np)
+            self.d_drivers.copy_(torch.from_numpy(self.drivers_np))
+            self._changes = None
+        self._table_rows = None
+
     # ------------------------------------------------------------------
     def get_state(self, b: int) -> dict:
         """Copy one instance state to host numpy arrays."""
@@ -503,6 +529,8 @@ class Batch:
             "ls": float(raw[lay["ls"]:lay["ls"] + 8].view(np.float64)[0]),
             "xb_bound": raw[lay["xb_bound"]:lay["xb_bound"] + 4 * X].view(np.int32).copy(),
             "xb_key": raw[lay["xb_key"]:lay["xb_key"] + 4 * X].view(np.int32).copy(),
+            "xb_near": raw[lay["xb_near"]:lay["xb_near"] + 4 * X].view(np.int32).copy(),
+            "xb_xdist": raw[lay["xb_xdist"]:lay["xb_xdist"] + 4 * X].view(np.float32).copy(),
             "thin_bound": raw[lay["thin_bound"]:lay["thin_bound"] + 4 * S].view(np.int32).copy(),
             "xb_state": raw[lay["xb_state"]:lay["xb_state"] + X].copy(),
             "tm": raw[lay["tm"]:lay["tm"] + S].copy(),
@@ -526,6 +554,9 @@ class Batch:
         put("ls", [st["ls"]], np.float64, 1)
         put("xb_bound", st["xb_bound"], np.int32, X)
         put("xb_key", st["xb_key"], np.int32, X)
+        # the last nearest-site update (the kernel writes these, never reads them)
+        put("xb_near", st.get("xb_near", np.zeros(X, np.int32)), np.int32, X)
+        put("xb_xdist", st.get("xb_xdist", np.zeros(X, np.float32)), np.float32, X)
         put("thin_bound", st["thin_bound"], np.int32, S)
         put("xb_state", st["xb_state"], np.uint8, X)
         put("tm", st["tm"], np.uint8, S)

@@ -16,7 +16,8 @@ def tab_abs(nb):
 def oracle_state_to_device(st: mo.State, topo) -> dict:
     key = mo.xb_keys(st.xb_xdist, st.xb_nearest, st.tm, topo)
     return dict(pos=st.x(), ls=st.ls, xb_bound=st.xb_bound.astype(np.int32),
-                xb_key=key.astype(np.int32), thin_bound=st.thin_bound.ravel().astype(np.int32),
+                xb_key=key.astype(np.int32), xb_near=st.xb_nearest.astype(np.int32),
+                xb_xdist=st.xb_xdist.astype(np.float32), thin_bound=st.thin_bound.ravel().astype(np.int32),
                 xb_state=st.xb_state.astype(np.uint8), tm=st.tm.ravel().astype(np.uint8))
 
 
