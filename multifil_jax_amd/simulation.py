@@ -195,7 +195,7 @@ class SimulationResult:
                    pCa=np.stack([r.pCa for r in results]),
                    lattice_spacing=np.stack([r.lattice_spacing for r in results]),
                    final_state=None, metadata=results[0].metadata, dt=results[0].dt,
-                   name=results[0].name,
+                   name=results[0].name + "_stacked",
                    grid_shape=metrics['axial_force'].shape[:-1],
                    axis_names=[axis_name] + results[0]._axis_names, coords=coords,
                    topology_config=results[0].topology_config)

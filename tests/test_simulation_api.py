@@ -114,6 +114,8 @@ def test_result_reductions_and_slicing():
         r.sel(nope=1)
     st = SimulationResult.stack([r, r], axis_name='lattice', axis_values=[2, 4])
     assert st.axial_force.shape == (2, 2, 3, 4) and st._axis_names[0] == 'lattice'
+    assert st.name == r.name + "_stacked"                # simulation.py:410
+    assert r.mean().name == r.name + "_mean" and r.std().name == r.name + "_std"
     assert abs(r.steady_state_force - float(np.mean(r.axial_force[..., -1:]))) < 1e-6
     assert "Mean force" in r.summary()
     assert r.metrics.axial_force is r.metrics['axial_force']
