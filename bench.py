@@ -152,13 +152,20 @@ def make_batch(B, T, lattice=2):
     return bt, topo, d, drv
 
 
-def plain_gpu(cfg, n_cond=8, n_steps=100, block=None):
-    """B = 1 per condition, one persistent-kernel launch per timestep."""
+def plain_gpu(cfg, B=256, n_steps=30, block=None):
+    """The plain per-condition form: each of the sweep's B conditions alone
+    (B = 1), one step-kernel launch per timestep, ``n_steps`` timed steps
+    each after an untimed equilibration and first step."""
     import torch
-    total = 0
+    from multifil_jax_amd.topology import SarcTopology
+    from multifil_jax_amd.params import get_skeletal_params
+    from multifil_jax_amd.engine import LatticePack, Batch
+    s, d = get_skeletal_params()
+    pack = LatticePack([SarcTopology.create(2, 2, s, d)])
+    drv = conditions(B, n_steps + 1)
     t_all = 0.0
-    for c in range(n_cond):
-        bt, *_ = make_batch(1, n_steps + 1)
+    for c in range(B):
+        bt = Batch(pack, np.zeros(1, np.int32), d.rows(1), drv[c:c + 1], seeds=np.array([c], np.uint64))
         bt.run(1, t0=0, cfg=cfg, block=block)          # init + first step, untimed
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -166,27 +173,53 @@ def plain_gpu(cfg, n_cond=8, n_steps=100, block=None):
             bt.run(1, t0=t, cfg=cfg, block=block, sync=False)
         torch.cuda.synchronize()
         t_all += time.perf_counter() - t0
-        total += n_steps
-    return total / t_all, f"{n_cond} conditions x {n_steps} steps, one launch per step"
+    return B * n_steps / t_all, f"all {B} conditions x {n_steps} steps each, one launch per step"
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it: model name, vendor, family /
+    model / stepping, the vector extensions the port can use, and the count
+    of logical CPUs listed."""
+    info, n = {}, 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "processor":
+                    n += 1
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    flags = set(info.get("flags", "").split())
+    ext = "/".join(e for e in ("avx2", "fma", "avx512f", "avx512_bf16") if e in flags) or "none"
+    return (f"{info.get('model name', 'unknown')} ({info.get('vendor_id', '?')} family {info.get('cpu family', '?')} "
+            f"model {info.get('model', '?')} stepping {info.get('stepping', '?')}; {n} logical CPUs listed; "
+            f"vector extensions {ext})")
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may run on
+    (sched_getaffinity), capped by OMP_NUM_THREADS when the host sets it
+    (the share of the machine this job is given)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    cap = int(env) if env and env.isdigit() and int(env) > 0 else aff
+    return max(1, min(aff, cap)), aff
 
 
 def cpu_baseline(B=256, T=1000, reps=3):
     """The C restatement of the reference algorithm (oracle/cpu_port.c: FP32,
     the reference solver's caps), OpenMP over conditions, on this host's
     cores: the sweep's own condition table (B conditions x T steps), one
-    warm-up, best of ``reps``.  Transition tables are built before timing,
-    as on the GPU."""
+    warm-up, best of ``reps``.  It is compiled here with -march=native
+    (falling back to the portable in-tree build if that fails).  Transition
+    tables are built before timing, as on the GPU.  The port computes the
+    force, residual, Newton count and n_bound per step, not the other 48
+    metrics: ``gpu_same_work`` times the GPU on that same work (reference
+    solver, no metrics row)."""
+    import tempfile
     from oracle import cpu_port
     from multifil_jax_amd.params import DYNAMIC_FIELDS
     from multifil_jax_amd.topology import SarcTopology
@@ -194,21 +227,43 @@ def cpu_baseline(B=256, T=1000, reps=3):
     s, d = get_skeletal_params()
     topo = SarcTopology.create(2, 2, s, d)
     p = {n: float(getattr(d, n)) for n in DYNAMIC_FIELDS}
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, aff = cpu_threads()
     drv = conditions(B, T)
     seeds = np.arange(B, dtype=np.uint64)
     prep = cpu_port.prepare(topo, p, drv)
-    cpu_port.run(topo, p, drv[:threads, :20], seeds[:threads], threads=threads,
+    try:
+        lib_path, build = cpu_port.build_native(tempfile.mkdtemp(prefix="mfc_")), "gcc -O3 -march=native -fopenmp"
+    except Exception as e:          # no compiler on the host: the portable build
+        lib_path, build = None, f"in-tree x86-64-v2 build (native build failed: {type(e).__name__})"
+    cpu_port.run(topo, p, drv[:threads, :20], seeds[:threads], threads=threads, lib_path=lib_path,
                  prepared={**prep, "tm": prep["tm"][:threads], "xb": prep["xb"][:threads]})
     best = None
     for _ in range(reps):
         t0 = time.perf_counter()
-        cpu_port.run(topo, p, drv, seeds, threads=threads, prepared=prep)
+        cpu_port.run(topo, p, drv, seeds, threads=threads, prepared=prep, lib_path=lib_path)
         el = time.perf_counter() - t0
         best = el if best is None else min(best, el)
     return {"value": B * T / best, "unit": "sarcomere-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cpus": aff, "cpu": cpu_model(), "build": build,
             "sample": f"the full sweep ({B} conditions x {T} steps), C restatement (oracle/cpu_port.c, FP32, "
-                      f"reference solver caps), OpenMP {threads} threads on {cpu_model()}, best of {reps}"}
+                      f"reference solver caps 4 x 6, force / residual / iterations / n_bound per step), "
+                      f"OpenMP {threads} threads ({aff} CPUs in the affinity mask, OMP_NUM_THREADS "
+                      f"{os.environ.get('OMP_NUM_THREADS', 'unset')}), best of {reps}"}
+
+
+def gpu_same_work(B=256, T=1000, W=20):
+    """The GPU on the CPU port's work: the same sweep, reference solver,
+    force only (no 52-metric row); sarcomere-steps/s over T steps."""
+    from multifil_jax_amd.engine import SolverConfig
+    import torch
+    bt, *_ = make_batch(B, W + T)
+    bt.d_metrics = None
+    cfg = SolverConfig(mode="reference")
+    bt.run(W, t0=0, cfg=cfg)
+    bt.prepare(T, t0=W)
+    torch.cuda.synchronize()
+    ms = _timed(bt, T, cfg, t0=W)
+    return B * T / (ms / 1e3)
 
 
 def _timed(bt, n_steps, cfg, t0=0):
@@ -460,6 +515,11 @@ def main():
         out["plain_gpu"] = {"value": pv, "unit": "sarcomere-steps/s", "sample": psample}
         out["batched_over_plain"] = value / pv
         out["cpu_baseline"] = cpu_baseline()
+        g = gpu_same_work()
+        out["cpu_baseline"]["gpu_same_work"] = {
+            "value": g, "unit": "sarcomere-steps/s",
+            "what": "the GPU on the port's work: same sweep, reference solver, force only",
+            "ratio": g / out["cpu_baseline"]["value"]}
     print(json.dumps(out))
 
 

@@ -23,15 +23,33 @@ class MfcParams(C.Structure):
 
 
 _lib = None
+_lib_path = None
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError(f"{LIB} is missing: run `make -C oracle`")
-        _lib = C.CDLL(LIB)
+def build_native(dest_dir: str) -> str:
+    """Compile cpu_port.c for the host it runs on (-O3 -march=native, OpenMP)
+    into dest_dir; returns the library path.  The in-tree libmfc.so is built
+    for a portable x86-64-v2 target instead."""
+    import subprocess
+    out = os.path.join(dest_dir, "libmfc_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c11", "-shared",
+           "-I", os.path.join(os.path.dirname(_HERE), "include"), os.path.join(_HERE, "cpu_port.c"),
+           "-o", out, "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return out
+
+
+def load(path: str = None):
+    """Load the port (the in-tree build, or ``path``); a later call with
+    another path switches libraries."""
+    global _lib, _lib_path
+    path = path or LIB
+    if _lib is None or _lib_path != path:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `make -C oracle`")
+        _lib = C.CDLL(path)
         _lib.mfc_run.restype = C.c_int
+        _lib_path = path
     return _lib
 
 
@@ -77,13 +95,13 @@ def prepare(topo, p: dict, drivers, dt=1.0):
     return dict(pack=LatticePack([topo]), tm=tm, xb=xb, P=P)
 
 
-def run(topo, p: dict, drivers, seeds, dt=1.0, n_newton=4, n_cg=6, threads=1, prepared=None):
+def run(topo, p: dict, drivers, seeds, dt=1.0, n_newton=4, n_cg=6, threads=1, prepared=None, lib_path=None):
     """B instances of one lattice with constant pCa / spacing per instance.
 
     drivers: (B, T, 3) float32 (z_line, pCa, spacing); seeds (B,) uint64.
     Returns force (B, T) float64, residual (B, T), iters (B, T), n_bound (B, T).
     """
-    lib = load()
+    lib = load(lib_path)
     drivers = np.ascontiguousarray(drivers, np.float32)
     B, T, _ = drivers.shape
     pr = prepared or prepare(topo, p, drivers, dt)
