@@ -7,7 +7,6 @@ import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-SRC = os.path.join(_HERE, "csrc", "mf_engine.hip")
 OUT = os.path.join(_HERE, "libmultifil_amd.so")
 
 HIP_FLAGS = [
@@ -24,28 +23,49 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build_engine(verbose: bool = False) -> str:
-    """Compile the library from source (always: no up-to-date check), and
-    print one line saying that it did, with the time taken and a digest of
-    the sources."""
+def _sources():
+    csrc = os.path.join(_HERE, "csrc")
+    return sorted(os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".hip"))
+
+
+def build_engine(verbose: bool = False, jobs: int = None) -> str:
+    """Compile the library from source (always: no up-to-date check), one
+    object per translation unit in parallel, then link; print one line
+    saying that it did, with the time taken and a digest of the sources."""
     import hashlib
+    import tempfile
     import time
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-8000:])
-    if verbose and r.stderr:
-        print(r.stderr[-4000:])
+    srcs = _sources()
+    jobs = jobs or max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 16))
+    inc = ["-I", os.path.join(ROOT, "include")]
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="mf_build_") as tmpd:
+        def compile_one(src):
+            obj = os.path.join(tmpd, os.path.basename(src) + ".o")
+            r = subprocess.run([hipcc(), *compile_flags, *inc, "-c", src, "-o", obj],
+                               capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n" + r.stderr[-8000:])
+            if verbose and r.stderr:
+                print(r.stderr[-4000:])
+            return obj
+        with ThreadPoolExecutor(jobs) as ex:
+            objs = list(ex.map(compile_one, srcs))
+        tmp = OUT + ".tmp"
+        r = subprocess.run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed:\n" + r.stderr[-8000:])
     os.replace(tmp, OUT)
     h = hashlib.sha256()
-    srcs = [os.path.join(_HERE, "csrc", f) for f in sorted(os.listdir(os.path.join(_HERE, "csrc")))]
-    for f in srcs + [os.path.join(ROOT, "include", "multifil_amd.h")]:
+    hdrs = [os.path.join(_HERE, "csrc", f) for f in sorted(os.listdir(os.path.join(_HERE, "csrc"))) if f.endswith(".h")]
+    for f in srcs + hdrs + [os.path.join(ROOT, "include", "multifil_amd.h")]:
         with open(f, "rb") as fh:
             h.update(fh.read())
     print(f"build: compiled {os.path.relpath(OUT, ROOT)} from source in {time.time() - t0:.1f} s "
-          f"(gfx950, sources sha256 {h.hexdigest()[:16]})", flush=True)
+          f"({len(srcs)} translation units, {jobs} jobs; gfx950, sources sha256 {h.hexdigest()[:16]})", flush=True)
     return OUT
 
 

@@ -81,3 +81,27 @@ def test_bad_block_is_rejected(lib, skeletal):
     topo = SarcTopology.create(2, 2, s, d)
     rc, _ = _lds(lib, [topo], _lib.MF_SOLVER_REFINED, 640)
     assert rc != 0
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("mode", [_lib.MF_SOLVER_REFERENCE, _lib.MF_SOLVER_REFINED])
+@pytest.mark.parametrize("case", ["8x8", "10x10", "lethocerus4x4", "drosophila4x6"])
+def test_large_lattice_variant_fits(lib, case, mode, dynamic):
+    """The lattices above 6x6 of the reference's dynamic-spacing benchmark
+    (examples/benchmarks/benchmark_dynamic_ls.py:136-137: up to 10x10) and
+    the larger flight-muscle lattices launch on the level-2 workspace
+    variants: positions and per-head state in device memory, only the
+    atomically updated site arrays in LDS."""
+    from multifil_jax_amd.params import get_skeletal_params, get_lethocerus_params, get_drosophila_params
+    fn, shape = {"8x8": (get_skeletal_params, (8, 8)), "10x10": (get_skeletal_params, (10, 10)),
+                 "lethocerus4x4": (get_lethocerus_params, (4, 4)),
+                 "drosophila4x6": (get_drosophila_params, (4, 6))}[case]
+    s, d = fn()
+    topo = SarcTopology.create(*shape, s, d)
+    assert default_block([topo]) == 1024
+    rc, lds = _lds(lib, [topo], mode, 1024, dynamic)
+    assert rc == 0, lib.mf_last_error().decode()
+    assert 0 < lds <= LDS_PER_CU
+    rc, ws = _lds(lib, [topo], mode, 1024, dynamic, ws=True)
+    assert rc == 0 and ws > (8 if mode == _lib.MF_SOLVER_REFINED else 4) * (
+        topo.n_thick * topo.n_crowns + topo.n_thin * topo.n_sites)   # positions in the workspace
