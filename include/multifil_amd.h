@@ -66,6 +66,13 @@ extern "C" {
 #define MF_LAUNCH_KINETICS_ONLY 2 /* each step is kinetics_step alone
                                (timestep.py:65-129): nearest sites, TM and
                                XB transitions; no solve, no metrics row    */
+#define MF_LAUNCH_REBUILD 8  /* an instance whose pCa or spacing driver
+                               changes from one step to the next rebuilds
+                               its transition tables inside the kernel
+                               (the rows that step reads), so a trace runs
+                               in one launch; without it the caller splits
+                               launches at driver changes and calls
+                               mf_tables between them                    */
 #define MF_LAUNCH_SOLVE_ONLY 4 /* each step is solve_equilibrium alone
                                (solver.py:663-771) at the step's Z-line and
                                spacing (dynamic: d_ref = spacing, the state's

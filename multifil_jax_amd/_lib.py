@@ -24,6 +24,7 @@ MF_INST_DYNAMIC = 2
 MF_LAUNCH_DYNAMIC = 1
 MF_LAUNCH_KINETICS_ONLY = 2
 MF_LAUNCH_SOLVE_ONLY = 4
+MF_LAUNCH_REBUILD = 8
 MF_POP_NONE = 0
 MF_POP_MEAN_FIELD = 1
 MF_POP_EXPLICIT = 2

@@ -61,7 +61,8 @@ def feasible_blocks(batch: Batch, mode: str) -> list:
         L.B = batch.B
         L.block = blk
         L.cfg = SolverConfig(mode=mode).ctype()
-        L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if batch.is_dynamic.any() else 0
+        # driver traces run on the dynamic-spacing kernels (in-kernel rebuilds)
+        L.launch_flags = _lib.MF_LAUNCH_DYNAMIC if (batch.is_dynamic.any() or batch.has_driver_changes()) else 0
         L.lattices_host = C.addressof(batch.pack.desc_host)
         L.n_pops = 1
         if batch.pops is not None:     # explicit populations widen the LDS TM rows

@@ -208,15 +208,32 @@ __device__ inline void tm_generator(const float* prm, T pCa, int cfg, T Q[4][4],
   put(3, 0, k30); put(3, 2, k32); put(3, 3, -(k30 + k32));
 }
 
-// 6x6 XB generator at axial offset x, spacing d, permissiveness perm
+// The eleven XB rates at axial offset x, spacing d, permissiveness perm
 // (transitions.py:632-804, rate_functions.py:318-753).
-template <typename T, bool ACC = false>
-__device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, bool absorbing,
-                                    T Q[6][6], T w = T(1)) {
+template <typename T>
+struct XbRates { T r01, r04, r05, r10, r12, r21, r23, r32, r34, r40, r50; };
+
+// The parts of the XB rates that depend on neither the head's offset nor
+// the spacing: kT and the calcium-dependent SRX rate r50.
+template <typename T>
+struct XbPre { T kt, r50; };
+
+template <typename T>
+__device__ inline XbPre<T> xb_pre(const float* prm, T pCa) {
+  XbPre<T> p;
+  p.kt = kT_pNnm<T>(T(prm[P_TEMP]));
+  const T ca = ca_conc<T>(pCa), b = prm[P_SRX_B];
+  const T cab = pow(ca, b);
+  p.r50 = T(prm[P_SRX_K0]) + ((T(prm[P_SRX_KMAX]) - T(prm[P_SRX_K0])) * cab) / (pow(T(prm[P_SRX_CA50]), b) + cab);
+  return p;
+}
+
+template <typename T>
+__device__ inline XbRates<T> xb_rates(const float* prm, const XbPre<T>& pre, T d, T x, T perm) {
   const T cap = T(RATE_CAP);
   const T r = sqrt(x * x + d * d);
   const T th = atan2(d, x);
-  const T kt = kT_pNnm<T>(T(prm[P_TEMP]));
+  const T kt = pre.kt;
   const T dgw = r - T(prm[P_G_REST_W]), dcw = th - T(prm[P_C_REST_W]);
   const T dgs = r - T(prm[P_G_REST_S]), dcs = th - T(prm[P_C_REST_S]);
   const T Ew = (T(0.5) * T(prm[P_G_K_W]) * (dgw * dgw) + T(0.5) * T(prm[P_C_K_W]) * (dcw * dcw)) / kt;
@@ -227,36 +244,77 @@ __device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, b
   const T Ut1 = T(prm[P_U_T1]) + Es;
   const T Ut2 = T(prm[P_U_T2]) + Es;
   const T fs = T(prm[P_G_K_S]) * dgs + (T(1) / r) * T(prm[P_C_K_S]) * dcs;
+  XbRates<T> k;
   T r01 = perm * T(prm[P_R01]) * exp(-Ew);
   if (isnan(r01)) r01 = T(0);
   T r10 = fmin(exp(log(r01 + T(0.005)) - (Udrx - Uloose)), cap);
   if (isnan(r10)) r10 = cap;
   T r12 = T(prm[P_R12]) * exp(fmin(Ediff, T(30)) / T(2));
   if (isnan(r12)) r12 = T(0);
-  const T r21 = fmin(exp(log(r12) - (Uloose - Ut1)), cap);
-  const T r23 = fmin(T(prm[P_R23]) * exp(-fs * T(prm[P_D23]) / kt), cap);
-  const T r32 = exp(fmin(log(r23 + T(1e-30)) + (Ut2 - Ut1), log(cap)));
-  const T r34 = fmin(T(prm[P_R34]) * exp(fs * T(prm[P_D34]) / kt), cap);
-  const T r40 = prm[P_R40], r04 = prm[P_R04], r05 = prm[P_R05];
-  const T ca = ca_conc<T>(pCa), b = prm[P_SRX_B];
-  const T cab = pow(ca, b);
-  const T r50 = T(prm[P_SRX_K0]) +
-                ((T(prm[P_SRX_KMAX]) - T(prm[P_SRX_K0])) * cab) / (pow(T(prm[P_SRX_CA50]), b) + cab);
-  if (!ACC) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-      for (int j = 0; j < 6; ++j) Q[i][j] = T(0);
+  k.r01 = r01;
+  k.r10 = r10;
+  k.r12 = r12;
+  k.r21 = fmin(exp(log(r12) - (Uloose - Ut1)), cap);
+  k.r23 = fmin(T(prm[P_R23]) * exp(-fs * T(prm[P_D23]) / kt), cap);
+  k.r32 = exp(fmin(log(k.r23 + T(1e-30)) + (Ut2 - Ut1), log(cap)));
+  k.r34 = fmin(T(prm[P_R34]) * exp(fs * T(prm[P_D34]) / kt), cap);
+  k.r40 = prm[P_R40];
+  k.r04 = prm[P_R04];
+  k.r05 = prm[P_R05];
+  k.r50 = pre.r50;
+  return k;
+}
+
+template <typename T>
+__device__ inline XbRates<T> xb_rates(const float* prm, T pCa, T d, T x, T perm) {
+  return xb_rates<T>(prm, xb_pre<T>(prm, pCa), d, x, perm);
+}
+
+// Entry (i, j) of the 6x6 XB generator from its rates; the absorbing variant
+// has rows 0 and 4 zeroed (transitions.py:1077).  Layout transitions.py:
+// 283-298, diagonals :789-794.
+template <typename T>
+__device__ __forceinline__ T xb_gen_entry(const XbRates<T>& k, int i, int j, bool absorbing) {
+  switch (i) {
+    case 0: return absorbing ? T(0) : j == 0 ? -(k.r01 + k.r04 + k.r05) : j == 1 ? k.r01 : j == 4 ? k.r04 : j == 5 ? k.r05 : T(0);
+    case 1: return j == 0 ? k.r10 : j == 1 ? -(k.r10 + k.r12) : j == 2 ? k.r12 : T(0);
+    case 2: return j == 1 ? k.r21 : j == 2 ? -(k.r21 + k.r23) : j == 3 ? k.r23 : T(0);
+    case 3: return j == 2 ? k.r32 : j == 3 ? -(k.r32 + k.r34) : j == 4 ? k.r34 : T(0);
+    case 4: return absorbing ? T(0) : j == 0 ? k.r40 : j == 4 ? -k.r40 : T(0);
+    default: return j == 0 ? k.r50 : j == 5 ? -k.r50 : T(0);
   }
-  auto put = [&](int i, int j, T v) { if (ACC) Q[i][j] += w * v; else Q[i][j] = v; };
-  if (!absorbing) {  // rows 0 and 4 zeroed in the absorbing variant (transitions.py:1077)
-    put(0, 0, -(r01 + r04 + r05)); put(0, 1, r01); put(0, 4, r04); put(0, 5, r05);
-    put(4, 0, r40); put(4, 4, -r40);
-  }
-  put(1, 0, r10); put(1, 1, -(r10 + r12)); put(1, 2, r12);
-  put(2, 1, r21); put(2, 2, -(r21 + r23)); put(2, 3, r23);
-  put(3, 2, r32); put(3, 3, -(r32 + r34)); put(3, 4, r34);
-  put(5, 0, r50); put(5, 5, -r50);
+}
+
+// Row r of the XB generator (r may differ between the lanes of a wave: the
+// row is picked by selects, not by a branch).
+template <typename T>
+__device__ __forceinline__ void xb_gen_row(const XbRates<T>& k, int r, bool absorbing, T out[6]) {
+  const T z = T(0);
+  const T d0 = absorbing ? z : -(k.r01 + k.r04 + k.r05), d1 = -(k.r10 + k.r12), d2 = -(k.r21 + k.r23);
+  const T d3 = -(k.r32 + k.r34), d4 = absorbing ? z : -k.r40, d5 = -k.r50;
+  const T a01 = absorbing ? z : k.r01, a04 = absorbing ? z : k.r04, a05 = absorbing ? z : k.r05;
+  const T a40 = absorbing ? z : k.r40;
+  out[0] = r == 0 ? d0 : r == 1 ? k.r10 : r == 4 ? a40 : r == 5 ? k.r50 : z;
+  out[1] = r == 0 ? a01 : r == 1 ? d1 : r == 2 ? k.r21 : z;
+  out[2] = r == 1 ? k.r12 : r == 2 ? d2 : r == 3 ? k.r32 : z;
+  out[3] = r == 2 ? k.r23 : r == 3 ? d3 : z;
+  out[4] = r == 0 ? a04 : r == 3 ? k.r34 : r == 4 ? d4 : z;
+  out[5] = r == 0 ? a05 : r == 5 ? d5 : z;
+}
+
+// 6x6 XB generator (transitions.py:632-804).  ACC: Q += w * generator (the
+// mean-field blend, transitions.py:925-975), else Q = generator.
+template <typename T, bool ACC = false>
+__device__ inline void xb_generator(const float* prm, T pCa, T d, T x, T perm, bool absorbing,
+                                    T Q[6][6], T w = T(1)) {
+  const XbRates<T> k = xb_rates<T>(prm, pCa, d, x, perm);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const T v = xb_gen_entry<T>(k, i, j, absorbing);
+      if (ACC) Q[i][j] += w * v; else Q[i][j] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -349,6 +407,156 @@ __device__ inline void expm_ref(T A[N][N]) {
     for (int j = 0; j < N; ++j) {
       const T v = A[i][j] / rs;
       A[i][j] = isnan(v) ? T(1) / T(N) : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same expm, computed by the four lanes of a quad: lane l owns rows
+// 2 min(l, 2) and 2 min(l, 2) + 1 (lane 3 repeats lane 2's rows), and a
+// matrix product takes the other lanes' rows by DPP quad broadcasts.  Every
+// row sees the operations of expm_ref in the same order (products with k
+// ascending, the same Gauss-Jordan steps), so the result is bit for bit
+// expm_ref's.  All four lanes of the quad must be active.
+// ---------------------------------------------------------------------------
+template <int Q>
+__device__ __forceinline__ double quad_bcast_d(double v) {
+  constexpr int C = Q | (Q << 2) | (Q << 4) | (Q << 6);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), C, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), C, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// C = A B for the row pair of this lane (A, C: this lane's rows; B: the
+// lane's rows of B).  Row k of B comes from quad lane k / 2, one row at a
+// time.  C may alias A or B only through the caller's copy.
+template <int K>
+__device__ __forceinline__ void quad_fma_row(const double A[2][6], const double B[2][6], double acc[2][6]) {
+  double b[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) b[j] = quad_bcast_d<K / 2>(B[K & 1][j]);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[i][j] = fma(A[i][K], b[j], acc[i][j]);
+  // keep the scheduler from hoisting the next row's broadcasts above these
+  // FMAs: all six rows in flight at once would not fit the register file
+  __builtin_amdgcn_sched_barrier(0);
+}
+__device__ __forceinline__ void quad_matmul6(const double A[2][6], const double B[2][6], double C[2][6]) {
+  double acc[2][6];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[i][j] = 0.0;
+  quad_fma_row<0>(A, B, acc);
+  quad_fma_row<1>(A, B, acc);
+  quad_fma_row<2>(A, B, acc);
+  quad_fma_row<3>(A, B, acc);
+  quad_fma_row<4>(A, B, acc);
+  quad_fma_row<5>(A, B, acc);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) C[i][j] = acc[i][j];
+}
+
+// A <- expm(A) for a 6x6 matrix held as row pairs by a quad; l = lane & 3.
+__device__ inline void expm_quad6(double A[2][6], int l) {
+  const int rp = l < 2 ? l : 2;  // row pair 0, 1 or 2
+  // infinity norm: row sums in order, then the maximum over the quad
+  double nrm = 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rs += fabs(A[i][j]);
+    nrm = fmax(nrm, rs);
+  }
+  nrm = fmax(fmax(quad_bcast_d<0>(nrm), quad_bcast_d<1>(nrm)), quad_bcast_d<2>(nrm));
+  int s = (int)ceilf(log2f((float)nrm / 0.5f + 1e-10f));
+  s = s < 0 ? 0 : s;
+  const double sc = ldexp(1.0, -s);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= sc;
+  const double b0 = 1.0, b1 = 1.0 / 2.0, b2 = 1.0 / 9.0, b3 = 1.0 / 72.0, b4 = 1.0 / 1008.0,
+               b5 = 1.0 / 30240.0, b6 = 1.0 / 1209600.0;
+  double A2[2][6], A4[2][6], A6[2][6];
+  quad_matmul6(A, A, A2);
+  quad_matmul6(A2, A2, A4);
+  quad_matmul6(A4, A2, A6);
+  // V over A6 and W over A4, entry by entry (each read once, before it is
+  // overwritten)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double id = (2 * rp + i == j) ? 1.0 : 0.0;
+      const double v = b0 * id + b2 * A2[i][j] + b4 * A4[i][j] + b6 * A6[i][j];
+      const double w = b1 * id + b3 * A2[i][j] + b5 * A4[i][j];
+      A6[i][j] = v;
+      A4[i][j] = w;
+    }
+  double (&V)[2][6] = A6;
+  double (&W)[2][6] = A4;
+  quad_matmul6(A, W, A2);  // U
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      W[i][j] = V[i][j] - A2[i][j];
+      A[i][j] = V[i][j] + A2[i][j];
+    }
+  // Gauss-Jordan: every lane scales the broadcast pivot row itself (the
+  // owner's arithmetic exactly), the owner keeps the scaled row
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double pw[6], pa[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int kr = k & 1;
+      pw[j] = (k >> 1) == 0 ? quad_bcast_d<0>(W[kr][j]) : (k >> 1) == 1 ? quad_bcast_d<1>(W[kr][j])
+                                                                        : quad_bcast_d<2>(W[kr][j]);
+      pa[j] = (k >> 1) == 0 ? quad_bcast_d<0>(A[kr][j]) : (k >> 1) == 1 ? quad_bcast_d<1>(A[kr][j])
+                                                                        : quad_bcast_d<2>(A[kr][j]);
+    }
+    const double inv = 1.0 / pw[k];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { pw[j] *= inv; pa[j] *= inv; }
+    // selects, not a branch: the owner lane of a row differs within the
+    // quad, and a lane-divergent branch here costs the register file dearly
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const bool own = 2 * rp + i == k;
+      const double f = W[i][k];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double w = fma(-f, pw[j], W[i][j]), a = fma(-f, pa[j], A[i][j]);
+        W[i][j] = own ? pw[j] : w;
+        A[i][j] = own ? pa[j] : a;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
+  for (int q = 0; q < nsq; ++q) {
+    quad_matmul6(A, A, A2);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) A[i][j] = A2[i][j];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rs += A[i][j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double v = A[i][j] / rs;
+      A[i][j] = isnan(v) ? 1.0 / 6.0 : v;
     }
   }
 }

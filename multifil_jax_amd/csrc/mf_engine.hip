@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void tables_kernel(const mf_launch L, int nb_m
   if (m == 0) { tab[0] = pCa; tab[1] = d; tab[2] = d; tab[3] = 1.0f; }
   const double dt = L.cfg.dt;
   if (m < ktm * 27) {
-    tm_table_entry(P, m / 27, (double)pCa, dt, m % 27, tab);
+    tm_table_entry(P, m / 27, (double)pCa, dt, m % 27, tab + TAB_TM);
     return;
   }
   int key = m - ktm * 27;

@@ -94,7 +94,7 @@ __device__ __forceinline__ Pops pops_of(const mf_launch& L, int b) {
 // expm of Q dt, stored as FP32 cumulative rows (transitions.py:434-513,
 // 99-190, 580-602).
 __device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pCa, double dt, int cfg,
-                                               float* tab) {
+                                               float* tmrows) {
   double Q[4][4];
   if (P.blend_tm) {
 #pragma unroll
@@ -111,7 +111,7 @@ __device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pC
 #pragma unroll
     for (int j = 0; j < 4; ++j) Q[i][j] *= dt;
   expm_ref<4, double>(Q);
-  float* row = tab + TAB_TM + (set * 27 + cfg) * 16;
+  float* row = tmrows + (set * 27 + cfg) * 16;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double c = 0.0;
@@ -164,16 +164,143 @@ __device__ __forceinline__ void xb_table_entry(const Pops& P, double pCa, double
   }
 }
 
-// All kxb * 2 nb XB (or absorbing) entries of one instance at spacing d,
-// strided over a workgroup.  Kept out of line: the FP64 6x6 expm needs far
-// more registers than the step kernel around it may hold.
+// The rows of one XB key's entry from the quad's generator rows A (this
+// lane's two rows, rp = min(l, 2)): scaled by dt, expm_quad6 (bit for bit
+// xb_table_entry's result), stored; l = lane & 3.
+__device__ __forceinline__ void xb_quad_finish(double A[2][6], double dt, int key, bool absorbing, float* xbrows,
+                                               float* absrows, int l) {
+  const int rp = l < 2 ? l : 2;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= dt;
+  expm_quad6(A, l);
+  if (l == 3) return;
+  if (!absorbing) {
+    float* row = xbrows + (int64_t)key * 48;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      float* r = row + (2 * rp + i) * 8;
+      double c = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) { c += A[i][j]; r[j] = (float)c; }
+      r[6] = 0.0f;
+      r[7] = 0.0f;
+    }
+  } else if (rp == 1) {  // rows 2 and 3: P[3][4], P[2][0], P[3][0]
+    float* row = absrows + (int64_t)key * 4;
+    row[0] = (float)A[1][4];
+    row[1] = (float)A[0][0];
+    row[2] = (float)A[1][0];
+    row[3] = 0.0f;
+  }
+}
+
+// One mean-field-blended XB key's entry computed by the four lanes of a
+// quad: the populations' generators accumulated in order, as
+// xb_generator<double, true> does; l = lane & 3.
+__device__ __forceinline__ void xb_blend_entry_quad(const Pops& P, double pCa, double d, double dt, double center,
+                                                    int key, int nb, bool absorbing, float* xbrows, float* absrows,
+                                                    int l) {
+  const double perm = (double)((key % (2 * nb)) / nb);
+  const int rp = l < 2 ? l : 2;
+  double A[2][6];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] = 0.0;
+  for (int k = 0; k < P.K; ++k) {
+    const XbRates<double> r = xb_rates<double>(P.prm + k * MF_N_PARAMS, pCa, d, center, perm);
+    const double w = (double)P.frac[k];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double g[6];
+      xb_gen_row<double>(r, 2 * rp + i, absorbing, g);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) A[i][j] += w * g[j];
+    }
+  }
+  xb_quad_finish(A, dt, key, absorbing, xbrows, absrows, l);
+}
+
+template <typename T>
+__device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
+  XbRates<T> o;
+  o.r01 = __shfl(r.r01, src); o.r04 = __shfl(r.r04, src); o.r05 = __shfl(r.r05, src);
+  o.r10 = __shfl(r.r10, src); o.r12 = __shfl(r.r12, src); o.r21 = __shfl(r.r21, src);
+  o.r23 = __shfl(r.r23, src); o.r32 = __shfl(r.r32, src); o.r34 = __shfl(r.r34, src);
+  o.r40 = __shfl(r.r40, src); o.r50 = __shfl(r.r50, src);
+  return o;
+}
+
+// The kxb * 2 nb XB (or absorbing) entries of one instance at spacing d.
+// With ``list`` only the listed keys (count of them) are rebuilt.  A wave
+// takes a chunk of up to 64 keys: one lane per key evaluates the rates (the
+// transcendental part, which the four lanes of a quad would otherwise
+// repeat), then the quads take the keys 16 at a time, the rates by lane
+// shuffles, for the generator rows and the quad expm.  The chunk is a
+// multiple of 16 chosen so that every wave has keys.  Kept out of line, so
+// the step kernel around it keeps its own register allocation (templated
+// on the block size, so each copy gets the register budget of the kernels
+// that call it).  All the workgroup's threads call it.
+template <int BS>
 __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float pCa, double d, float dt,
                                                            const float* centers, int nb, int kxb,
                                                            bool absorbing, float* xbrows, float* absrows,
-                                                           int tid, int nthreads) {
-  for (int key = tid; key < kxb * 2 * nb; key += nthreads)
-    xb_table_entry(P, (double)pCa, (double)d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
-                   xbrows, absrows);
+                                                           int tid, int nthreads, const unsigned short* list,
+                                                           int count) {
+  const int l = tid & 3;
+  const int n = list ? count : kxb * 2 * nb;
+  if (P.blend_xb) {
+    for (int i = tid >> 2; i < n; i += nthreads >> 2) {
+      const int key = list ? list[i] : i;
+      xb_blend_entry_quad(P, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absorbing, xbrows,
+                          absrows, l);
+    }
+    return;
+  }
+  const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6, rp = l < 2 ? l : 2;
+  // lane s < kxb holds parameter set s's key-independent terms
+  const XbPre<double> pre = xb_pre<double>(P.prm + (lane < kxb ? lane : kxb - 1) * MF_N_PARAMS, (double)pCa);
+  const int per = (n + nw - 1) / nw;
+  const int chunk = per >= 64 ? 64 : (per + 15) & ~15;
+  for (int base = wave * chunk; base < n; base += nw * chunk) {
+    const int i = base + lane;
+    const bool mine = lane < chunk && i < n;
+    const int key = mine ? (list ? list[i] : i) : 0;
+    const int set = key / (2 * nb), k2 = key % (2 * nb);
+    XbPre<double> ps;
+    ps.kt = __shfl(pre.kt, set);
+    ps.r50 = __shfl(pre.r50, set);
+    XbRates<double> r = {};
+    if (mine) r = xb_rates<double>(P.prm + set * MF_N_PARAMS, ps, d, (double)centers[key % nb], (double)(k2 / nb));
+    for (int sub = 0; sub * 16 < chunk; ++sub) {
+      const int src = sub * 16 + (lane >> 2);
+      const XbRates<double> rk = shfl_rates(r, src);
+      const int kk = __shfl(key, src);
+      if (base + src < n) {
+        double A[2][6];
+#pragma unroll
+        for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, absorbing, A[i2]);
+        xb_quad_finish(A, (double)dt, kk, absorbing, xbrows, absrows, l);
+      }
+    }
+  }
+}
+
+// The keys marked in a 64-word bitmap, in increasing order, into list[]; run
+// by one whole wave (lane w takes word w); returns the count to every lane.
+__device__ __forceinline__ int compact_keys(const LDS uint32_t* bits, unsigned short* list, int lane) {
+  uint32_t w = bits[lane];
+  const float c = (float)__popc(w);
+  const float incl = wave_prefix(c, lane);
+  int at = (int)wave_prefix_excl(incl, lane);
+  while (w) {
+    const int b = __ffs(w) - 1;
+    list[at++] = (unsigned short)(lane * 32 + b);
+    w &= w - 1;
+  }
+  return (int)read_lane(incl, 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,6 +399,8 @@ struct Shared {
   LDS double* fil;       // [Tk][NF] metric per-filament values
   LDS float* ktip;       // [Tk] titin tip stiffness of each thick filament (last residual)
   LDS int* cnt;          // [NW][NC] metric counters, one row per wave
+  LDS uint32_t* need;    // [2][64] bitmaps of the XB keys a step reads (sampling, absorbing),
+                         // then the two compacted key counts
 };
 
 // LDS / workspace carve-up.  One routine computes the byte offsets (host:
@@ -279,7 +408,7 @@ struct Shared {
 // pointers), so the two cannot disagree.  Offsets of "Big" arrays are into
 // the workspace slice when GW is set.
 struct LdsOffsets {
-  int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, cbn, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum,
+  int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, cbn, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum, need,
       edges, prm, red, wpart, fil, ktip, cnt, total, ws_total;
 };
 template <int BS, bool REF64, int GW = 0>
@@ -321,6 +450,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.fil = take(8LL * d.Tk * NF);
   L.ktip = take(4LL * d.Tk);
   L.cnt = take(4 * NC * NW);
+  L.need = take(4 * (2 * 64 + 2));
   L.total = o;
   L.ws_total = w;
   return L;
@@ -370,6 +500,7 @@ __device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, con
   t.fil = (LDS double*)(base + o.fil);
   t.ktip = (LDS float*)(base + o.ktip);
   t.cnt = (LDS int*)(base + o.cnt);
+  t.need = (LDS uint32_t*)(base + o.need);
   return t;
 }
 
@@ -1259,6 +1390,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     sh.tnext[i] = (signed char)(li[G.tm_next + i] - k);
   }
   for (int i = tid; i < NC * NW; i += BS) sh.cnt[i] = 0;
+  for (int i = tid; i < 2 * 64 + 2; i += BS) sh.need[i] = 0;
 
   // per-XB (thin, face) slot with the target-valid flag; displacement origins
   for (int j = tid; j < X; j += BS)
@@ -1401,6 +1533,13 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   bool fpos_ready = false;  // fpos already holds this step's candidate positions
   // the split halves of a step (public launch flags, uniform per launch)
   const bool kin_only = (L.launch_flags & MF_LAUNCH_KINETICS_ONLY) != 0;
+  // in-kernel table rebuild on driver changes; the tables hold the drivers
+  // of row t_first when the launch starts
+  // (only the dynamic-spacing variants carry the out-of-line rebuild: a call
+  // inside the time loop would change the register allocation of the
+  // fixed-spacing kernels; the host runs traces on these variants)
+  const bool trace = DYN && (L.launch_flags & MF_LAUNCH_REBUILD) != 0;
+  float tab_pca = d_p, tab_ls = d_l;
   const bool solve_only = (L.launch_flags & MF_LAUNCH_SOLVE_ONLY) != 0;
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
     // The thread index is made opaque once per step: otherwise the compiler
@@ -1420,6 +1559,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     // old key and state of each head, for the dynamic-spacing metrics (their
     // absorbing chain is evaluated at the solved spacing)
     int q_kold[MX], q_sold[MX];
+    bool chg = false;  // this step rebuilds a fixed-spacing instance's tables
     if (!kin) {
       // Nearest update, then a solve against the parameter row's z_line and
       // lattice_spacing (the reference's run() leaves those at their defaults).
@@ -1458,13 +1598,23 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       z_prev = zl;
       z_solve = zl;
       d_solve = lsd;
-      if (DYN && rad.on) {
-        // kinetics at the carried spacing (simulation.py:691-693): rebuild
-        // the XB sampling rows; d_ref = ls (z0 / z)^nu (simulation.py:694-696)
-        xb_table_rebuild(pops, pCa, ls_carry, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid, BS);
+      // d_ref = ls (z0 / z)^nu (simulation.py:694-696)
+      if (DYN && rad.on) rad.dref = lsd * powf(z_trace0 / zl, I.nu);
+      // A fixed-spacing instance whose pCa or spacing differs from the one
+      // its tables hold rebuilds them here (the reference forms its
+      // generators from each step's drivers, transitions.py:889-912): the TM
+      // rows now, the XB rows this step reads after the nearest-site search.
+      // (dynamic spacing rebuilds its XB rows every step; its TM rows follow pCa)
+      const bool new_pca = trace && pCa != tab_pca;
+      chg = trace && !(DYN && rad.on) && (new_pca || lsd != tab_ls);
+      if (DYN && (chg || (rad.on && new_pca))) {
+        for (int e = tid; e < dm.ktm * 27; e += BS)
+          tm_table_entry(pops, e / 27, (double)pCa, (double)L.cfg.dt, e % 27, (float*)sh.tmcum);
+        tab_pca = pCa;
+        tab_ls = lsd;
         __syncthreads();
-        rad.dref = lsd * powf(z_trace0 / zl, I.nu);
       }
+      const bool rebuild = DYN && (rad.on || chg);
       // thin.axial += dz (simulation.py:683-687)
       if (dz != 0.0f) {
         for (int i = dm.nk + tid; i < dm.N; i += BS) {
@@ -1545,12 +1695,22 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           q_near[q] += thin * dm.ns;  // thin-global site index from here on
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
           const int lb = xlab ? xlab[j] * 2 * nb : 0;
-          const float* r0 = xbtab + (int64_t)(lb + bn) * 48 + old * 8;
-          const float* r1 = xbtab + (int64_t)(lb + nb + bn) * 48 + old * 8;
-          q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
-          q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
-          if (!(DYN && rad.on) && (old == 2 || old == 3)) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
-          if (DYN) { q_kold[q] = keyold; q_sold[q] = old; }
+          if (rebuild) {
+            // the rows this step reads: both permissiveness levels of the
+            // head's bin, and the absorbing row of its previous key
+            atomicOr((uint32_t*)(sh.need + ((lb + bn) >> 5)), 1u << ((lb + bn) & 31));
+            atomicOr((uint32_t*)(sh.need + ((lb + nb + bn) >> 5)), 1u << ((lb + nb + bn) & 31));
+            if (old == 2 || old == 3)
+              atomicOr((uint32_t*)(sh.need + 64 + (keyold >> 5)), 1u << (keyold & 31));
+          } else {
+            const float* r0 = xbtab + (int64_t)(lb + bn) * 48 + old * 8;
+            const float* r1 = xbtab + (int64_t)(lb + nb + bn) * 48 + old * 8;
+            q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
+            q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
+            if (!(DYN && rad.on) && (old == 2 || old == 3)) q_ab[q] = *(const float4*)(abstab + (int64_t)keyold * 4);
+          }
+          q_kold[q] = keyold;
+          q_sold[q] = old;
           q_bin[q] = lb + bn;
           q_occ[q] = sh.tbound[q_near[q]] >= 0;
           // The owner of this head's own bound site before any K2 write.
@@ -1567,6 +1727,45 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       }
       __syncthreads();
       tmc ^= 1;
+      if (DYN && rebuild) {
+        // the marked XB rows at the kinetics spacing (dynamic: the carried
+        // one, simulation.py:691-693), and for a changed fixed-spacing
+        // instance the marked absorbing rows too, then the prefetch
+        const double d_kin = (DYN && rad.on) ? ls_carry : (double)lsd;
+        // the marked keys as compact lists (in the CG buffer, idle during
+        // the kinetics), so the busy quads are contiguous
+        // (one list per kind, 2 bytes per key; the CG buffer holds 2 N; a
+        // second list that does not fit rebuilds every absorbing row)
+        unsigned short* ks = (unsigned short*)sh.pv;
+        const bool two = 2 * dm.kxb * 2 * nb <= 2 * dm.N;
+        unsigned short* ka = two ? ks + dm.kxb * 2 * nb : nullptr;
+        if (wave == 0) {
+          const int ns_ = compact_keys(sh.need, ks, lane);
+          const int na_ = (chg && two) ? compact_keys(sh.need + 64, ka, lane) : 0;
+          if (lane == 0) { sh.need[128] = (uint32_t)ns_; sh.need[129] = (uint32_t)na_; }
+        }
+        __syncthreads();
+        const int n_s = (int)sh.need[128], n_a = (int)sh.need[129];
+        xb_table_rebuild<BS>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid, BS,
+                             ks, n_s);
+        if (chg)
+          xb_table_rebuild<BS>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS,
+                               ka, n_a);
+        __syncthreads();
+        for (int i = tid; i < (chg ? 128 : 64); i += BS) sh.need[i] = 0;
+#pragma unroll
+        for (int q = 0; q < MX; ++q) {
+          const int j = tid + q * BS;
+          if (j < X) {
+            const int old = q_old[q];
+            const float* r0 = xbtab + (int64_t)q_bin[q] * 48 + old * 8;
+            const float* r1 = xbtab + (int64_t)(q_bin[q] + nb) * 48 + old * 8;
+            q_r0[q] = *(const float4*)r0; q_s0[q] = *(const float2*)(r0 + 4);
+            q_r1[q] = *(const float4*)r1; q_s1[q] = *(const float2*)(r1 + 4);
+            if (chg && (old == 2 || old == 3)) q_ab[q] = *(const float4*)(abstab + (int64_t)q_kold[q] * 4);
+          }
+        }
+      }
       const auto tmc_p = sh.tm + tmc * S;
       // pre-solve thick positions the metrics need (work, energy changes).
       // Written after K1's barrier: wave 0 may still be reading the previous
@@ -1665,8 +1864,16 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     if (DYN && rad.on && !solve_only) {
       // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
       // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
-      xb_table_rebuild(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS);
+      unsigned short* ka = (unsigned short*)sh.pv;   // the CG buffer is idle after the solve
+      if (wave == 0) {
+        const int na_ = compact_keys(sh.need + 64, ka, lane);
+        if (lane == 0) sh.need[129] = (uint32_t)na_;
+      }
       __syncthreads();
+      xb_table_rebuild<BS>(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS,
+                           ka, (int)sh.need[129]);
+      __syncthreads();
+      for (int i = 64 + tid; i < 128; i += BS) sh.need[i] = 0;
       float m_atpe = 0.0f, m_tear = 0.0f;
 #pragma unroll
       for (int q = 0; q < MX; ++q) {

@@ -418,6 +418,10 @@ class Batch:
         torch = _torch()
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
+    def has_driver_changes(self) -> bool:
+        """Whether any instance's pCa or spacing changes within the trace."""
+        return len(self.segments(0, self.T)) > 1 if self.T > 0 else False
+
     def segments(self, t0: int, n: int) -> List[tuple]:
         """Split [t0, t0+n) where any instance's pCa or spacing changes: the
         transition tables are rebuilt between segments."""
@@ -429,18 +433,27 @@ class Batch:
         cuts = [t0] + [int(c) for c in ch] + [t0 + n]
         return [(a, b - a) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
-    def prepare(self, n_steps: int, t0: int = 0):
+    def _split(self, split: Optional[bool]) -> bool:
+        # default: one launch per constant stretch for an all-fixed-spacing
+        # batch (the tables kernel spreads each rebuild over the whole chip,
+        # where an in-kernel rebuild runs on the instance's own workgroup),
+        # in-kernel rebuilds when dynamic instances rebuild every step anyway
+        return (not self.is_dynamic.any()) if split is None else bool(split)
+
+    def prepare(self, n_steps: int, t0: int = 0, split: Optional[bool] = None):
         """Build and upload the instance tables that run(n_steps, t0) will
         use (and find the driver segments), so a timed run starts with every
         input already on the device."""
         init = ~self.initialized
-        for (s0, n) in (self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]):
+        split = self._split(split)
+        segs = self.segments(t0, n_steps) if (n_steps > 0 and split) else [(t0, n_steps)]
+        for (s0, n) in segs:
             self._instances(n, init, s0)
             init = np.zeros_like(init)
 
     def run(self, n_steps: int, t0: int = 0, cfg: SolverConfig = None, uniforms=None,
             block: Optional[int] = None, sync: bool = True, kernel_events=None, noalloc: bool = False,
-            launch_flags: int = 0):
+            launch_flags: int = 0, split: Optional[bool] = None):
         """Advance every instance from step t0 by n_steps (the first call
         equilibrates each instance before its first step, as the reference's
         create_and_equilibrate does).
@@ -454,8 +467,15 @@ class Batch:
         batch allocates.
         launch_flags: extra MF_LAUNCH_* bits (MF_LAUNCH_KINETICS_ONLY,
         MF_LAUNCH_SOLVE_ONLY: each step runs that half of the step alone).
+        split: cut the run into one launch per stretch of constant pCa and
+        spacing, with the tables rebuilt by mf_tables between launches
+        (True), or run one launch in which each instance rebuilds its own
+        tables when its drivers change (False, MF_LAUNCH_REBUILD).  Both give
+        the same bits.  None: split unless the batch holds dynamic-spacing
+        instances.
         """
         cfg = cfg or SolverConfig()
+        split = self._split(split)
         if block is not None:
             self.block = int(block)
         if t0 < 0 or t0 + n_steps > self.T:
@@ -470,13 +490,18 @@ class Batch:
                 raise ValueError(f"uniforms last dim {u_stride} < {need}")
             uniforms = uniforms.contiguous()
         stream = self._stream()
-        segs = self.segments(t0, n_steps) if n_steps > 0 else [(t0, 0)]
+        changes = n_steps > 0 and len(self.segments(t0, n_steps)) > 1
+        segs = self.segments(t0, n_steps) if (n_steps > 0 and split) else [(t0, n_steps)]
         for (s0, n) in segs:
             init = ~self.initialized
             self._instances(n, init, s0)
             L = self._launch(cfg, uniforms, u_stride)
             L.t_base = int(s0)
             L.launch_flags |= int(launch_flags)
+            if changes and not split:
+                # in-kernel table rebuilds run on the dynamic-spacing kernels
+                # (fixed-spacing instances keep their fixed spacing there)
+                L.launch_flags |= _lib.MF_LAUNCH_REBUILD | _lib.MF_LAUNCH_DYNAMIC
             rows = self.drivers_np[:, s0, 1:3]
             if self._table_rows is None or not np.array_equal(rows, self._table_rows) \
                     or cfg.dt != self._table_dt:
@@ -494,6 +519,8 @@ class Batch:
             if kernel_events is not None and s0 == segs[-1][0]:
                 kernel_events[1].record(_torch().cuda.current_stream(self.device))
             self.initialized[:] = True
+        if changes and not split:
+            self._table_rows = None     # the kernel left the tables at other drivers
         if sync:
             _torch().cuda.synchronize(self.device)
 

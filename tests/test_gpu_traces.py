@@ -109,3 +109,38 @@ def test_nan_driver_sentinel(torch_cuda, topo22, skeletal):
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][1], runs[1][1])
     assert not np.isnan(runs[0][1]).any()
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_in_kernel_rebuild_equals_split_launches(torch_cuda, topo22, skeletal, dynamic):
+    """A pCa twitch and a spacing ramp in one launch, each instance
+    rebuilding its own tables in the kernel when its drivers change
+    (MF_LAUNCH_REBUILD), give the same bits as one launch per constant
+    stretch with mf_tables between them (three instances: a twitch, a
+    spacing ramp, a constant one).  With dynamic spacing the kernel
+    rebuilds at every step either way; there the check covers the
+    rebuild of only the rows a step reads."""
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    _, d = skeletal
+    T = 60
+    drv = np.zeros((3, T, 3), np.float32)
+    drv[:, :, 0] = 1000.0
+    drv[:, :, 2] = 14.0
+    drv[0, :, 1] = _twitch(T + 20)[20:]
+    drv[1, :, 1] = 4.8
+    drv[1, :, 2] = np.linspace(13.5, 14.5, T, dtype=np.float32)
+    drv[2, :, 1] = 5.2
+    K = 5.0 * topo22.n_thick
+    out = []
+    for split in (False, True):
+        print(f"twitch / spacing ramp, split={split}, dynamic={dynamic}", flush=True)
+        bt = Batch(LatticePack([topo22]), np.zeros(3, np.int32), d.rows(3), drv, seeds=np.arange(3, dtype=np.uint64),
+                   dynamic=(np.full(3, K, np.float32), np.zeros(3, np.float32)) if dynamic else None)
+        bt.run(T, cfg=SolverConfig(mode="refined"), split=split)
+        out.append((bt.force(), bt.metrics(), [bt.get_state(b) for b in range(3)]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
+    for a, b in zip(out[0][2], out[1][2]):
+        for k in a:
+            assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    assert out[0][1][0, :, 5].max() > 0          # the twitch binds heads
