@@ -647,6 +647,13 @@ __device__ __forceinline__ void precond(const float (&r)[MS], float (&z)[MS], co
 #pragma unroll
   for (int s = 0; s < MS; ++s) z[s] = sl.node[s] >= 0 ? r[s] : 0.0f;
   chain_scan<MS>(z, sl, lane, 1, 0);
+  // A chain's padding slots (past its last node: lanes 43-63 of an 86-site
+  // thin chain) leave the first scan holding its running total; zeroed, so
+  // the second scan sums the chain's nodes only.  Left in, they add
+  // (padding count) x (sum of r) / k to every node of the chain: still SPD,
+  // but a far worse preconditioner (about 1.5 more CG passes per solve).
+#pragma unroll
+  for (int s = 0; s < MS; ++s) z[s] = sl.node[s] >= 0 ? z[s] : 0.0f;
   chain_scan<MS>(z, sl, lane, 0, 1);
 #pragma unroll
   for (int s = 0; s < MS; ++s) z[s] *= sl.thick[s] ? inv_kk : inv_kn;
@@ -762,6 +769,34 @@ __device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& 
   ktt = (float)(-T(N_TITIN) * (Fp * ad * ad / (Lq * Lq) + F * d * d / (Lq * Lq * Lq)));
 }
 
+// A site's list of bound heads, relinked into increasing head order.  The
+// heads insert themselves with atomicExch in the order their waves happen
+// to arrive, which varies from run to run, and the solve sums a site's
+// partner forces and stiffness terms in list order: unsorted, the solved
+// positions would vary in their last bits between identical runs.  Run by
+// the lane that owns the site node, the only lane that walks the list
+// during the solve; lists hold a few heads.
+template <bool REF64, int GW>
+__device__ __forceinline__ void sort_site_list(const Shared<REF64, GW>& sh, int site) {
+  int j = sh.shead[site];
+  if (j < 0 || sh.xnext[j] < 0) return;
+  int first = -1;
+  while (j >= 0) {
+    const int nx = sh.xnext[j];
+    if (first < 0 || j < first) {
+      sh.xnext[j] = (short)first;
+      first = j;
+    } else {
+      int q = first;
+      for (int r = sh.xnext[q]; r >= 0 && r < j; r = sh.xnext[q]) q = r;
+      sh.xnext[j] = sh.xnext[q];
+      sh.xnext[q] = (short)j;
+    }
+    j = nx;
+  }
+  sh.shead[site] = first;
+}
+
 // Partners of the slot nodes for the current bindings (start of a solve).
 template <int MS, bool REF64, int GW>
 __device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, const Dims& dm, const Slots<MS>& sl,
@@ -774,6 +809,7 @@ __device__ __forceinline__ void build_partners(const Shared<REF64, GW>& sh, cons
     for (int h = 0; h < Partners<MS>::P; ++h) { pt.pk[s][h] = -1; pt.k[s][h] = 0.0f; }
     const int n = sl.node[s];
     if (n < 0) continue;
+    if (!sl.thick[s]) sort_site_list(sh, n - dm.nk);
     if (PM == 0) {
       pt.overflow[s] = !sl.thick[s];  // sites walk their whole list
       continue;

@@ -117,9 +117,13 @@ def test_in_kernel_rebuild_equals_split_launches(torch_cuda, topo22, skeletal, d
     rebuilding its own tables in the kernel when its drivers change
     (MF_LAUNCH_REBUILD), give the same bits as one launch per constant
     stretch with mf_tables between them (three instances: a twitch, a
-    spacing ramp, a constant one).  With dynamic spacing the kernel
-    rebuilds at every step either way; there the check covers the
+    spacing ramp, a constant one).  Both runs use the dynamic-spacing
+    kernel variants, which carry the in-kernel rebuild, so they differ only
+    in where the tables come from (another kernel variant may round the
+    solve differently in the last FP64 bits).  With dynamic spacing the
+    kernel rebuilds at every step either way; there the check covers the
     rebuild of only the rows a step reads."""
+    from multifil_jax_amd import _lib
     from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
     _, d = skeletal
     T = 60
@@ -136,7 +140,8 @@ def test_in_kernel_rebuild_equals_split_launches(torch_cuda, topo22, skeletal, d
         print(f"twitch / spacing ramp, split={split}, dynamic={dynamic}", flush=True)
         bt = Batch(LatticePack([topo22]), np.zeros(3, np.int32), d.rows(3), drv, seeds=np.arange(3, dtype=np.uint64),
                    dynamic=(np.full(3, K, np.float32), np.zeros(3, np.float32)) if dynamic else None)
-        bt.run(T, cfg=SolverConfig(mode="refined"), split=split)
+        bt.run(T, cfg=SolverConfig(mode="refined"), split=split,
+               launch_flags=_lib.MF_LAUNCH_DYNAMIC if split else 0)
         out.append((bt.force(), bt.metrics(), [bt.get_state(b) for b in range(3)]))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
