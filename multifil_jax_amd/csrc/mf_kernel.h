@@ -239,16 +239,12 @@ __device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
 // transcendental part, which the four lanes of a quad would otherwise
 // repeat), then the quads take the keys 16 at a time, the rates by lane
 // shuffles, for the generator rows and the quad expm.  The chunk is a
-// multiple of 16 chosen so that every wave has keys.  Kept out of line, so
-// the step kernel around it keeps its own register allocation (templated
-// on the block size, so each copy gets the register budget of the kernels
-// that call it).  All the workgroup's threads call it.
-template <int BS>
-__device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float pCa, double d, float dt,
-                                                           const float* centers, int nb, int kxb,
-                                                           bool absorbing, float* xbrows, float* absrows,
-                                                           int tid, int nthreads, const unsigned short* list,
-                                                           int count) {
+// multiple of 16 chosen so that every wave has keys.  All the workgroup's
+// threads call it.
+__device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, double d, float dt,
+                                                      const float* centers, int nb, int kxb, bool absorbing,
+                                                      float* xbrows, float* absrows, int tid, int nthreads,
+                                                      const unsigned short* list, int count) {
   const int l = tid & 3;
   const int n = list ? count : kxb * 2 * nb;
   if (P.blend_xb) {
@@ -286,6 +282,36 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
       }
     }
   }
+}
+
+// The rebuild out of line, so the step kernel around it keeps its own
+// register allocation (templated on the block size, so each copy gets the
+// register budget of the kernels that call it).
+template <int BS>
+__device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float pCa, double d, float dt,
+                                                           const float* centers, int nb, int kxb,
+                                                           bool absorbing, float* xbrows, float* absrows,
+                                                           int tid, int nthreads, const unsigned short* list,
+                                                           int count) {
+  xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list, count);
+}
+
+// The step kernel's rebuild: a call for the LDS-resident variants, inline
+// for the global-workspace ones.  Values live across a call sit in the
+// callee-saved registers or are spilled, and at 12 node slots per lane the
+// compiler then reloads the CG state from scratch on every CG pass
+// (~180 scratch operations per pass at 6x6, against ~90 inline): a 6x6
+// dynamic-spacing step takes 1.26 ms inline and 1.6 ms with the call.  At
+// 2-8 slots per lane the call is as fast or faster (4x4: 331 against
+// 368 us per dynamic step).
+template <int BS, int GW>
+__device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d, float dt, const float* centers,
+                                             int nb, int kxb, bool absorbing, float* xbrows, float* absrows,
+                                             int tid, const unsigned short* list, int count) {
+  if constexpr (GW > 0)
+    xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count);
+  else
+    xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count);
 }
 
 // The keys marked in a 64-word bitmap, in increasing order, into list[]; run
@@ -1782,10 +1808,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         __syncthreads();
         const int n_s = (int)sh.need[128], n_a = (int)sh.need[129];
-        xb_table_rebuild<BS>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid, BS,
+        rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid,
                              ks, n_s);
         if (chg)
-          xb_table_rebuild<BS>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS,
+          rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
                                ka, n_a);
         __syncthreads();
         for (int i = tid; i < (chg ? 128 : 64); i += BS) sh.need[i] = 0;
@@ -1906,7 +1932,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         if (lane == 0) sh.need[129] = (uint32_t)na_;
       }
       __syncthreads();
-      xb_table_rebuild<BS>(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid, BS,
+      rebuild_rows<BS, GW>(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
                            ka, (int)sh.need[129]);
       __syncthreads();
       for (int i = 64 + tid; i < 128; i += BS) sh.need[i] = 0;
