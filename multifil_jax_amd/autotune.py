@@ -32,8 +32,8 @@ from . import _lib
 from .engine import Batch, SolverConfig, default_block
 
 BLOCKS = (512, 768, 1024)
-N_CG = (4, 6, 8, 10, 12)
-CG_FORCING = (0.0, 0.003)
+N_CG = (3, 4, 6, 8, 12)
+CG_FORCING = (0.0, 0.03, 0.3)
 
 _cache: Dict[tuple, Tuple[int, int, float]] = {}
 

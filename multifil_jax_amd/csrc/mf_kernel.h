@@ -239,8 +239,15 @@ __device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
 // transcendental part, which the four lanes of a quad would otherwise
 // repeat), then the quads take the keys 16 at a time, the rates by lane
 // shuffles, for the generator rows and the quad expm.  The chunk is a
-// multiple of 16 chosen so that every wave has keys.  All the workgroup's
+// multiple of 16 chosen so that every wave has keys.  The quads of a wave
+// run their squaring loops in lockstep, so a sub-round lasts as long as its
+// key with the most squarings (about a third of the keys have a rate at
+// the cap and need 16 at dt = 1 ms, the others 1-11): the chunk's keys are
+// ranked by their scaling power s and the quads take them in that order
+// (SORT; the inline copy in the workspace variants skips the ranking, whose
+// registers cost those variants more than it saves).  All the workgroup's
 // threads call it.
+template <bool SORT>
 __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, double d, float dt,
                                                       const float* centers, int nb, int kxb, bool absorbing,
                                                       float* xbrows, float* absrows, int tid, int nthreads,
@@ -270,11 +277,34 @@ __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, 
     ps.r50 = __shfl(pre.r50, set);
     XbRates<double> r = {};
     if (mine) r = xb_rates<double>(P.prm + set * MF_N_PARAMS, ps, d, (double)centers[key % nb], (double)(k2 / nb));
+    // s from the infinity norm of Q dt, twice the largest exit rate times dt
+    // (the rule of expm_quad6); lanes without a key rank last
+    int sb = 19;
+    if (SORT && mine) {
+      double ex = fmax(fmax(r.r10 + r.r12, r.r21 + r.r23), fmax(r.r32 + r.r34, r.r50));
+      if (!absorbing) ex = fmax(ex, fmax(r.r01 + r.r04 + r.r05, r.r40));
+      const int sv = (int)ceilf(log2f((float)(2.0 * ex * (double)dt) / 0.5f + 1e-10f));
+      sb = sv < 0 ? 0 : (sv > 18 ? 18 : sv);
+    }
+    // counting sort of the chunk by s (ballots), then the inverse
+    // permutation: lane p learns which lane holds the key of rank p
+    const uint64_t below = (1ull << lane) - 1ull;
+    int rank = 0;
+    if constexpr (SORT) {
+#pragma unroll
+      for (int v = 0; v <= 19; ++v) {
+        const uint64_t m = __ballot(sb == v);
+        rank += v < sb ? __popcll(m) : (v == sb ? __popcll(m & below) : 0);
+      }
+    }
+    const int holder = SORT ? __builtin_amdgcn_ds_permute(rank << 2, lane) : lane;
+    const int n_mine = __popcll(__ballot(mine));
     for (int sub = 0; sub * 16 < chunk; ++sub) {
-      const int src = sub * 16 + (lane >> 2);
+      const int pos = sub * 16 + (lane >> 2);
+      const int src = SORT ? __shfl(holder, pos) : pos;
       const XbRates<double> rk = shfl_rates(r, src);
       const int kk = __shfl(key, src);
-      if (base + src < n) {
+      if (pos < n_mine) {
         double A[2][6];
 #pragma unroll
         for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, absorbing, A[i2]);
@@ -293,7 +323,8 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
                                                            bool absorbing, float* xbrows, float* absrows,
                                                            int tid, int nthreads, const unsigned short* list,
                                                            int count) {
-  xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list, count);
+  xb_table_rebuild_body<true>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list,
+                              count);
 }
 
 // The step kernel's rebuild: a call for the LDS-resident variants, inline
@@ -309,7 +340,8 @@ __device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d,
                                              int nb, int kxb, bool absorbing, float* xbrows, float* absrows,
                                              int tid, const unsigned short* list, int count) {
   if constexpr (GW > 0)
-    xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count);
+    xb_table_rebuild_body<false>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list,
+                                 count);
   else
     xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count);
 }
@@ -1758,10 +1790,9 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
           const int bn = bin_of(sh.edges, nb, inv_w, xd);
           const int lb = xlab ? xlab[j] * 2 * nb : 0;
           if (rebuild) {
-            // the rows this step reads: both permissiveness levels of the
-            // head's bin, and the absorbing row of its previous key
-            atomicOr((uint32_t*)(sh.need + ((lb + bn) >> 5)), 1u << ((lb + bn) & 31));
-            atomicOr((uint32_t*)(sh.need + ((lb + nb + bn) >> 5)), 1u << ((lb + nb + bn) & 31));
+            // the absorbing row of the head's previous key; its sampling
+            // row is marked after this barrier, once its site's new TM
+            // state (the permissiveness) is known
             if (old == 2 || old == 3)
               atomicOr((uint32_t*)(sh.need + 64 + (keyold >> 5)), 1u << (keyold & 31));
           } else {
@@ -1801,6 +1832,21 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         unsigned short* ks = (unsigned short*)sh.pv;
         const bool two = 2 * dm.kxb * 2 * nb <= 2 * dm.N;
         unsigned short* ka = two ? ks + dm.kxb * 2 * nb : nullptr;
+        // the one sampling row each head reads: its bin at its nearest
+        // site's permissiveness after this step's TM transitions (K2's rule)
+        {
+          const auto tnew = sh.tm + tmc * S;
+#pragma unroll
+          for (int q = 0; q < MX; ++q) {
+            const int j = tid + q * BS;
+            if (j < X) {
+              const bool perm = (tnew[q_near[q]] == 3) && (sh.xslot[j] < 0);
+              const int key = (perm ? nb : 0) + q_bin[q];
+              atomicOr((uint32_t*)(sh.need + (key >> 5)), 1u << (key & 31));
+            }
+          }
+        }
+        __syncthreads();
         if (wave == 0) {
           const int ns_ = compact_keys(sh.need, ks, lane);
           const int na_ = (chg && two) ? compact_keys(sh.need + 64, ka, lane) : 0;

@@ -46,7 +46,7 @@ def test_run_dynamic_spacing(torch_cuda, topo22, skeletal):
 
 
 def test_autotune_picks_a_converging_config(torch_cuda, topo22, skeletal):
-    from multifil_jax_amd.autotune import tune, feasible_blocks
+    from multifil_jax_amd.autotune import tune, feasible_blocks, N_CG, CG_FORCING
     from multifil_jax_amd.engine import LatticePack, Batch
     _, d = skeletal
     B, T = 32, 40
@@ -56,7 +56,7 @@ def test_autotune_picks_a_converging_config(torch_cuda, topo22, skeletal):
     blocks = feasible_blocks(bt, "refined")
     assert 768 in blocks and 1024 in blocks
     blk, cfg = tune(bt, mode="refined", trial_steps=10, warmup_steps=5)
-    assert blk in blocks and cfg.n_cg in (4, 6, 8, 10, 12)
+    assert blk in blocks and cfg.n_cg in N_CG and cfg.cg_forcing in CG_FORCING
     assert not bt.initialized.any()           # the tuner ran on a copy
     bt.block = blk
     bt.run(T, cfg=cfg)
