@@ -341,6 +341,13 @@ class Batch:
         self.d_ws = None
         self._changes = None
         self.block = default_block(pack.topos)
+        if self.is_dynamic.any() and self.block == 768:
+            # the dynamic-spacing kernels need more registers: at 512 threads
+            # (256 VGPRs per lane) a 2x2 dynamic step takes 155 us against
+            # 178 us at 768, where a variant fits the lattices
+            from .autotune import feasible_blocks
+            if 512 in feasible_blocks(self, "refined"):
+                self.block = 512
         self.initialized = np.zeros(self.B, dtype=bool)
         self._table_rows = None      # (B, 2) pCa / spacing the tables were built for
         self._table_dt = None
