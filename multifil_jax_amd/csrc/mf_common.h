@@ -337,8 +337,9 @@ __device__ __forceinline__ void matmul(const T A[N][N], const T B[N][N], T C[N][
 // reference computes it; Pade-type rational approximant with the reference's
 // coefficients; (V-U) X = (V+U) by Gauss-Jordan without pivoting (V-U =
 // I - A_s/2 + O(A_s^2) is diagonally dominant for ||A_s|| <= 1/2);
-// min(s, 18) squarings; rows renormalised; NaN -> 1/N.
-template <int N, typename T>
+// min(s, 18) squarings; with RENORM (a generator's exponential: a
+// stochastic matrix) rows renormalised and NaN -> 1/N.
+template <int N, typename T, bool RENORM = true>
 __device__ inline void expm_ref(T A[N][N]) {
   T nrm = T(0);
 #pragma unroll
@@ -398,6 +399,7 @@ __device__ inline void expm_ref(T A[N][N]) {
 #pragma unroll
       for (int j = 0; j < N; ++j) A[i][j] = A2[i][j];
   }
+  if (!RENORM) return;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     T rs = T(0);
@@ -409,6 +411,87 @@ __device__ inline void expm_ref(T A[N][N]) {
       A[i][j] = isnan(v) ? T(1) / T(N) : v;
     }
   }
+}
+
+// The three entries of exp(Q_abs dt) the metrics use (P[3][4], P[2][0],
+// P[3][0]; Q_abs: the XB generator with rows 0 and 4 zeroed, states 0 and 4
+// absorbing, transitions.py:1029-1078), from its transient block.  States
+// 1-3 leak only to 0 (r10, from 1) and to 4 (r34, from 3) and never reach
+// 5, so with A = T dt (T: the rates among states 1-3, as in Q_abs) the
+// absorption probabilities within dt are A^-1 (e^A - I) (R dt), R the two
+// exit columns: a 3x3 exponential and one 3x3 solve with two right-hand
+// sides (partial pivoting) instead of a 6x6 exponential.  The same numbers
+// to FP64 rounding; A is nonsingular (r10 >= the 0.005-floored exit).
+// a = A (3x3, already times dt), e10 = r10 dt, e34 = r34 dt.
+template <typename T>
+__device__ inline void absorbing_probs(const T a[3][3], T e10, T e34, T out[3]) {
+  T E[3][3], M[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { E[i][j] = a[i][j]; M[i][j] = a[i][j]; }
+  expm_ref<3, T, false>(E);
+  // right-hand sides (E - I) e0 e10 and (E - I) e2 e34
+  T y[3], z[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    y[i] = (E[i][0] - (i == 0 ? T(1) : T(0))) * e10;
+    z[i] = (E[i][2] - (i == 2 ? T(1) : T(0))) * e34;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // partial pivoting among rows k..2 (selects, no branches)
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i) {
+      const bool sw = fabs(M[i][k]) > fabs(M[k][k]);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const T u = M[k][j], v = M[i][j];
+        M[k][j] = sw ? v : u;
+        M[i][j] = sw ? u : v;
+      }
+      const T uy = y[k], vy = y[i], uz = z[k], vz = z[i];
+      y[k] = sw ? vy : uy; y[i] = sw ? uy : vy;
+      z[k] = sw ? vz : uz; z[i] = sw ? uz : vz;
+    }
+    const T inv = T(1) / M[k][k];
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i) {
+      const T f = M[i][k] * inv;
+#pragma unroll
+      for (int j = k; j < 3; ++j) M[i][j] = fma(-f, M[k][j], M[i][j]);
+      y[i] = fma(-f, y[k], y[i]);
+      z[i] = fma(-f, z[k], z[i]);
+    }
+  }
+#pragma unroll
+  for (int k = 2; k >= 0; --k) {
+#pragma unroll
+    for (int j = k + 1; j < 3; ++j) {
+      y[k] = fma(-M[k][j], y[j], y[k]);
+      z[k] = fma(-M[k][j], z[j], z[k]);
+    }
+    y[k] /= M[k][k];
+    z[k] /= M[k][k];
+  }
+  out[0] = z[2];  // P[3][4]
+  out[1] = y[1];  // P[2][0]
+  out[2] = y[2];  // P[3][0]
+}
+
+// The transient block of a rate set, times dt, accumulated with weight w
+// (ACC: the mean-field blend, transitions.py:925-975) or assigned.
+template <typename T, bool ACC>
+__device__ __forceinline__ void absorbing_block(const XbRates<T>& k, T dt, T w, T a[3][3], T& e10, T& e34) {
+  const T b[3][3] = {{-(k.r10 + k.r12), k.r12, T(0)},
+                     {k.r21, -(k.r21 + k.r23), k.r23},
+                     {T(0), k.r32, -(k.r32 + k.r34)}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a[i][j] = ACC ? a[i][j] + w * (b[i][j] * dt) : b[i][j] * dt;
+  e10 = ACC ? e10 + w * (k.r10 * dt) : k.r10 * dt;
+  e34 = ACC ? e34 + w * (k.r34 * dt) : k.r34 * dt;
 }
 
 // ---------------------------------------------------------------------------

@@ -120,6 +120,37 @@ __device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pC
   }
 }
 
+// The absorbing-chain entry of XB key (set * 2 nb + perm * nb + bin) at
+// spacing d: P[3][4], P[2][0], P[3][0] of the chain with states 0 and 4
+// absorbing (transitions.py:1029-1078), from its 3x3 transient block
+// (absorbing_probs); mean-field populations blend the block.
+__device__ __forceinline__ void absorbing_entry(const Pops& P, double pCa, double d, double dt, double center,
+                                                int key, int nb, float* absrows) {
+  const int set = key / (2 * nb), k2 = key % (2 * nb);
+  const double perm = (double)(k2 / nb);
+  double a[3][3], e10 = 0.0, e34 = 0.0;
+  if (P.blend_xb) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) a[i][j] = 0.0;
+    for (int k = 0; k < P.K; ++k) {
+      const XbRates<double> r = xb_rates<double>(P.prm + k * MF_N_PARAMS, pCa, d, center, perm);
+      absorbing_block<double, true>(r, dt, (double)P.frac[k], a, e10, e34);
+    }
+  } else {
+    const XbRates<double> r = xb_rates<double>(P.prm + set * MF_N_PARAMS, pCa, d, center, perm);
+    absorbing_block<double, false>(r, dt, 1.0, a, e10, e34);
+  }
+  double o[3];
+  absorbing_probs<double>(a, e10, e34, o);
+  float* row = absrows + (int64_t)key * 4;
+  row[0] = (float)o[0];
+  row[1] = (float)o[1];
+  row[2] = (float)o[2];
+  row[3] = 0.0f;
+}
+
 // XB key (set * 2 nb + perm * nb + bin) at spacing d: the 6 cumulative rows
 // of the sampling table (transitions.py:632-922), or, for the absorbing
 // chain (rows 0 and 4 zeroed, transitions.py:1029-1078), the three entries
@@ -127,9 +158,13 @@ __device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pC
 // (transitions.py:925-975).
 __device__ __forceinline__ void xb_table_entry(const Pops& P, double pCa, double d, double dt, double center,
                                                int key, int nb, bool absorbing, float* xbrows, float* absrows) {
-  double Q[6][6];
   const int set = key / (2 * nb), k2 = key % (2 * nb);
   const double perm = (double)(k2 / nb);
+  if (absorbing) {
+    absorbing_entry(P, pCa, d, dt, center, key, nb, absrows);
+    return;
+  }
+  double Q[6][6];
   if (P.blend_xb) {
 #pragma unroll
     for (int i = 0; i < 6; ++i)
@@ -254,6 +289,14 @@ __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, 
                                                       const unsigned short* list, int count) {
   const int l = tid & 3;
   const int n = list ? count : kxb * 2 * nb;
+  if (absorbing) {
+    // one lane per key: a 3x3 exponential (absorbing_entry)
+    for (int i = tid; i < n; i += nthreads) {
+      const int key = list ? list[i] : i;
+      absorbing_entry(P, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absrows);
+    }
+    return;
+  }
   if (P.blend_xb) {
     for (int i = tid >> 2; i < n; i += nthreads >> 2) {
       const int key = list ? list[i] : i;

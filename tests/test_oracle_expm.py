@@ -96,3 +96,34 @@ def test_pade_solve_needs_no_pivoting(topo22, pdict):
             assert np.all(piv == 0)
             f = M[:, k + 1:, k] / M[:, k, k][:, None]
             M[:, k + 1:, :] -= f[:, :, None] * M[:, k, None, :]
+
+
+def test_absorbing_chain_from_its_transient_block(topo22, pdict):
+    """The metrics' absorbing chain (states 0 and 4 absorbing,
+    transitions.py:1029-1078) needs P[3][4], P[2][0] and P[3][0] of
+    exp(Q_abs dt).  The device computes them from the 3x3 transient block:
+    with A = T dt (rates among states 1-3) and exits r10 (1 -> 0), r34
+    (3 -> 4), the absorption probabilities within dt are
+    A^-1 (e^A - I) (R dt).  Checked here against scipy's 6x6 exponential of
+    the absorbing generator over every bin, both permissiveness levels,
+    three spacings and two calcium levels (a sign or an index slip in the
+    block fails by orders of magnitude)."""
+    c = np.asarray(topo22.xb_bin_centers, np.float64)
+    worst = 0.0
+    for pCa in (4.5, 6.5):
+        for d in (11.0, 14.0, 16.5):
+            for perm in (0.0, 1.0):
+                Q = mo.absorbing(mo.xb_generators(pdict, pCa, d, c, perm=perm))
+                for q in Q:
+                    P = sla.expm(q * 1.0)
+                    A = q[1:4, 1:4] * 1.0
+                    e10, e34 = q[1, 0] * 1.0, q[3, 4] * 1.0
+                    E = sla.expm(A)
+                    y = np.linalg.solve(A, (E[:, 0] - np.eye(3)[:, 0]) * e10)
+                    z = np.linalg.solve(A, (E[:, 2] - np.eye(3)[:, 2]) * e34)
+                    got = np.array([z[2], y[1], y[2]])
+                    want = np.array([P[3, 4], P[2, 0], P[3, 0]])
+                    assert np.all(got >= -1e-15)
+                    worst = max(worst, float(np.max(np.abs(got - want))))
+                    assert np.allclose(got, want, rtol=1e-9, atol=1e-15), (pCa, d, perm, got, want)
+    assert worst < 1e-12
