@@ -68,9 +68,11 @@ __device__ __forceinline__ float philox_uniform(uint64_t seed, int step, int pur
 // block m of purpose 0 and lanes 2-3 block m of purpose 1, and every lane
 // takes its word idx%4 of each by a quad broadcast from lanes 0 and 2.
 // All four lanes of every quad must be active.
+// (a quad permutation reads a valid lane for every lane, so the move needs
+// no "old" operand and mov_dpp spares the copy of one into the destination)
 template <int CTRL>
 __device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, false);
 }
 __device__ __forceinline__ void philox_uniform_pair(uint64_t seed, int step, int idx, float& u0, float& u1) {
   const int r = idx & 3;
@@ -326,9 +328,9 @@ __device__ __forceinline__ void matmul(const T A[N][N], const T B[N][N], T C[N][
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      T acc = T(0);
+      T acc = A[i][0] * B[0][j];
 #pragma unroll
-      for (int k = 0; k < N; ++k) acc = fma(A[i][k], B[k][j], acc);
+      for (int k = 1; k < N; ++k) acc = fma(A[i][k], B[k][j], acc);
       C[i][j] = acc;
     }
 }
@@ -505,8 +507,8 @@ __device__ __forceinline__ void absorbing_block(const XbRates<T>& k, T dt, T w, 
 template <int Q>
 __device__ __forceinline__ double quad_bcast_d(double v) {
   constexpr int C = Q | (Q << 2) | (Q << 4) | (Q << 6);
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), C, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), C, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), C, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), C, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 
@@ -521,17 +523,13 @@ __device__ __forceinline__ void quad_fma_row(const double A[2][6], const double 
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc[i][j] = fma(A[i][K], b[j], acc[i][j]);
+    for (int j = 0; j < 6; ++j) acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
   // keep the scheduler from hoisting the next row's broadcasts above these
   // FMAs: all six rows in flight at once would not fit the register file
   __builtin_amdgcn_sched_barrier(0);
 }
 __device__ __forceinline__ void quad_matmul6(const double A[2][6], const double B[2][6], double C[2][6]) {
   double acc[2][6];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) acc[i][j] = 0.0;
   quad_fma_row<0>(A, B, acc);
   quad_fma_row<1>(A, B, acc);
   quad_fma_row<2>(A, B, acc);
@@ -624,7 +622,13 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
     __builtin_amdgcn_sched_barrier(0);
   }
   const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
-  for (int q = 0; q < nsq; ++q) {
+  // squarings in pairs, A -> A2 -> A, so no copy between them
+  int q = 0;
+  for (; q + 2 <= nsq; q += 2) {
+    quad_matmul6(A, A, A2);
+    quad_matmul6(A2, A2, A);
+  }
+  if (q < nsq) {
     quad_matmul6(A, A, A2);
 #pragma unroll
     for (int i = 0; i < 2; ++i)

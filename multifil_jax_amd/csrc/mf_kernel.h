@@ -269,24 +269,29 @@ __device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
 }
 
 // The kxb * 2 nb XB (or absorbing) entries of one instance at spacing d.
-// With ``list`` only the listed keys (count of them) are rebuilt.  A wave
-// takes a chunk of up to 64 keys: one lane per key evaluates the rates (the
-// transcendental part, which the four lanes of a quad would otherwise
-// repeat), then the quads take the keys 16 at a time, the rates by lane
-// shuffles, for the generator rows and the quad expm.  The chunk is a
-// multiple of 16 chosen so that every wave has keys.  The quads of a wave
-// run their squaring loops in lockstep, so a sub-round lasts as long as its
-// key with the most squarings (about a third of the keys have a rate at
-// the cap and need 16 at dt = 1 ms, the others 1-11): the chunk's keys are
-// ranked by their scaling power s and the quads take them in that order
-// (SORT; the inline copy in the workspace variants skips the ranking, whose
-// registers cost those variants more than it saves).  All the workgroup's
-// threads call it.
-template <bool SORT>
+// With ``list`` only the listed keys (count of them) are rebuilt.  Sampling
+// rows, in three passes over the listed keys:
+//  1. one lane per key evaluates the rates (the transcendental part, which
+//     the four lanes of a quad would otherwise repeat) and parks them in the
+//     key's own row slot, with the key's scaling power s: twice the largest
+//     exit rate times dt, the infinity norm of Q dt that expm_quad6 scales
+//     by (a third of the keys have a rate at the cap and need 16 squarings
+//     at dt = 1 ms, the others 1-11);
+//  2. (with ``order``) a counting sort of the keys by s, largest first,
+//     through the 20 counters of ``hist``;
+//  3. the quads take the keys 16 at a time (a sub-round of one wave, whose
+//     quads square in lockstep, so it lasts as long as its largest s) and
+//     write the rows over the parked rates.  Sub-round j of the sorted list
+//     goes to wave j mod nw, in reverse order on every other pass (a snake),
+//     so the waves' sums of sub-round costs even out.
+// Each key's result depends only on its own rates, not on the order.  All
+// the workgroup's threads call it; ``hist`` must be zero on entry and is
+// zero again on return.
 __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, double d, float dt,
                                                       const float* centers, int nb, int kxb, bool absorbing,
                                                       float* xbrows, float* absrows, int tid, int nthreads,
-                                                      const unsigned short* list, int count) {
+                                                      const unsigned short* list, int count,
+                                                      unsigned short* order, uint32_t* hist) {
   const int l = tid & 3;
   const int n = list ? count : kxb * 2 * nb;
   if (absorbing) {
@@ -308,51 +313,63 @@ __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, 
   const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6, rp = l < 2 ? l : 2;
   // lane s < kxb holds parameter set s's key-independent terms
   const XbPre<double> pre = xb_pre<double>(P.prm + (lane < kxb ? lane : kxb - 1) * MF_N_PARAMS, (double)pCa);
-  const int per = (n + nw - 1) / nw;
-  const int chunk = per >= 64 ? 64 : (per + 15) & ~15;
-  for (int base = wave * chunk; base < n; base += nw * chunk) {
-    const int i = base + lane;
-    const bool mine = lane < chunk && i < n;
-    const int key = mine ? (list ? list[i] : i) : 0;
+  // 1. rates and s per key, parked in the key's slot (11 doubles, then s)
+  for (int b0 = 0; b0 < n; b0 += nthreads) {
+    const int i = b0 + tid;
+    const int key = i < n ? (list ? list[i] : i) : 0;
     const int set = key / (2 * nb), k2 = key % (2 * nb);
     XbPre<double> ps;
     ps.kt = __shfl(pre.kt, set);
     ps.r50 = __shfl(pre.r50, set);
-    XbRates<double> r = {};
-    if (mine) r = xb_rates<double>(P.prm + set * MF_N_PARAMS, ps, d, (double)centers[key % nb], (double)(k2 / nb));
-    // s from the infinity norm of Q dt, twice the largest exit rate times dt
-    // (the rule of expm_quad6); lanes without a key rank last
-    int sb = 19;
-    if (SORT && mine) {
-      double ex = fmax(fmax(r.r10 + r.r12, r.r21 + r.r23), fmax(r.r32 + r.r34, r.r50));
-      if (!absorbing) ex = fmax(ex, fmax(r.r01 + r.r04 + r.r05, r.r40));
-      const int sv = (int)ceilf(log2f((float)(2.0 * ex * (double)dt) / 0.5f + 1e-10f));
-      sb = sv < 0 ? 0 : (sv > 18 ? 18 : sv);
-    }
-    // counting sort of the chunk by s (ballots), then the inverse
-    // permutation: lane p learns which lane holds the key of rank p
-    const uint64_t below = (1ull << lane) - 1ull;
-    int rank = 0;
-    if constexpr (SORT) {
-#pragma unroll
-      for (int v = 0; v <= 19; ++v) {
-        const uint64_t m = __ballot(sb == v);
-        rank += v < sb ? __popcll(m) : (v == sb ? __popcll(m & below) : 0);
+    if (i < n) {
+      const XbRates<double> r =
+          xb_rates<double>(P.prm + set * MF_N_PARAMS, ps, d, (double)centers[key % nb], (double)(k2 / nb));
+      double* slot = (double*)(xbrows + (int64_t)key * 48);
+      slot[0] = r.r01; slot[1] = r.r04; slot[2] = r.r05; slot[3] = r.r10; slot[4] = r.r12; slot[5] = r.r21;
+      slot[6] = r.r23; slot[7] = r.r32; slot[8] = r.r34; slot[9] = r.r40; slot[10] = r.r50;
+      if (order) {
+        const double ex = fmax(fmax(fmax(r.r10 + r.r12, r.r21 + r.r23), fmax(r.r32 + r.r34, r.r50)),
+                               fmax(r.r01 + r.r04 + r.r05, r.r40));
+        const int sv = (int)ceilf(log2f((float)(2.0 * ex * (double)dt) / 0.5f + 1e-10f));
+        const int sb = sv < 0 ? 0 : (sv > 19 ? 19 : sv);
+        ((int*)slot)[22] = sb;
+        atomicAdd(hist + sb, 1u);
       }
     }
-    const int holder = SORT ? __builtin_amdgcn_ds_permute(rank << 2, lane) : lane;
-    const int n_mine = __popcll(__ballot(mine));
-    for (int sub = 0; sub * 16 < chunk; ++sub) {
-      const int pos = sub * 16 + (lane >> 2);
-      const int src = SORT ? __shfl(holder, pos) : pos;
-      const XbRates<double> rk = shfl_rates(r, src);
-      const int kk = __shfl(key, src);
-      if (pos < n_mine) {
-        double A[2][6];
+  }
+  __syncthreads();
+  // 2. counting sort, largest s first
+  if (order) {
+    if (tid < 20) {
+      uint32_t off = 0;
+      for (int b = 19; b > tid; --b) off += hist[b];
+      hist[20 + tid] = off;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nthreads) {
+      const int key = list ? list[i] : i;
+      const int sb = ((const int*)(xbrows + (int64_t)key * 48))[22];
+      order[atomicAdd(hist + 20 + sb, 1u)] = (unsigned short)key;
+    }
+    if (tid < 20) hist[tid] = 0u;
+    __syncthreads();
+  }
+  // 3. the quads, sub-round by sub-round in snake order
+  const int rounds = (n + 15) >> 4;
+  for (int pass = 0;; ++pass) {
+    const int j = pass * nw + ((pass & 1) ? nw - 1 - wave : wave);
+    if (j >= rounds) break;
+    const int pos = j * 16 + (lane >> 2);
+    if (pos < n) {
+      const int kk = order ? order[pos] : (list ? list[pos] : pos);
+      const double* slot = (const double*)(xbrows + (int64_t)kk * 48);
+      XbRates<double> rk;
+      rk.r01 = slot[0]; rk.r04 = slot[1]; rk.r05 = slot[2]; rk.r10 = slot[3]; rk.r12 = slot[4]; rk.r21 = slot[5];
+      rk.r23 = slot[6]; rk.r32 = slot[7]; rk.r34 = slot[8]; rk.r40 = slot[9]; rk.r50 = slot[10];
+      double A[2][6];
 #pragma unroll
-        for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, absorbing, A[i2]);
-        xb_quad_finish(A, (double)dt, kk, absorbing, xbrows, absrows, l);
-      }
+      for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, absorbing, A[i2]);
+      xb_quad_finish(A, (double)dt, kk, absorbing, xbrows, absrows, l);
     }
   }
 }
@@ -365,9 +382,9 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
                                                            const float* centers, int nb, int kxb,
                                                            bool absorbing, float* xbrows, float* absrows,
                                                            int tid, int nthreads, const unsigned short* list,
-                                                           int count) {
-  xb_table_rebuild_body<true>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list,
-                              count);
+                                                           int count, unsigned short* order, uint32_t* hist) {
+  xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list, count,
+                        order, hist);
 }
 
 // The step kernel's rebuild: a call for the LDS-resident variants, inline
@@ -381,12 +398,14 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
 template <int BS, int GW>
 __device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d, float dt, const float* centers,
                                              int nb, int kxb, bool absorbing, float* xbrows, float* absrows,
-                                             int tid, const unsigned short* list, int count) {
+                                             int tid, const unsigned short* list, int count,
+                                             unsigned short* order, uint32_t* hist) {
   if constexpr (GW > 0)
-    xb_table_rebuild_body<false>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list,
-                                 count);
+    xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count, order,
+                          hist);
   else
-    xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count);
+    xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count, order,
+                         hist);
 }
 
 // The keys marked in a 64-word bitmap, in increasing order, into list[]; run
@@ -501,7 +520,8 @@ struct Shared {
   LDS float* ktip;       // [Tk] titin tip stiffness of each thick filament (last residual)
   LDS int* cnt;          // [NW][NC] metric counters, one row per wave
   LDS uint32_t* need;    // [2][64] bitmaps of the XB keys a step reads (sampling, absorbing),
-                         // then the two compacted key counts
+                         // then the two compacted key counts, then the rebuild's
+                         // [20] counters and [20] cursors of its sort by squaring count
 };
 
 // LDS / workspace carve-up.  One routine computes the byte offsets (host:
@@ -551,7 +571,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.fil = take(8LL * d.Tk * NF);
   L.ktip = take(4LL * d.Tk);
   L.cnt = take(4 * NC * NW);
-  L.need = take(4 * (2 * 64 + 2));
+  L.need = take(4 * (2 * 64 + 2 + 40));
   L.total = o;
   L.ws_total = w;
   return L;
@@ -1527,7 +1547,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
     sh.tnext[i] = (signed char)(li[G.tm_next + i] - k);
   }
   for (int i = tid; i < NC * NW; i += BS) sh.cnt[i] = 0;
-  for (int i = tid; i < 2 * 64 + 2; i += BS) sh.need[i] = 0;
+  for (int i = tid; i < 2 * 64 + 2 + 40; i += BS) sh.need[i] = 0;
 
   // per-XB (thin, face) slot with the target-valid flag; displacement origins
   for (int j = tid; j < X; j += BS)
@@ -1897,11 +1917,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         }
         __syncthreads();
         const int n_s = (int)sh.need[128], n_a = (int)sh.need[129];
+        // the keys sorted by their squaring count go after the two lists,
+        // when the buffer holds a third
+        unsigned short* ko = 3 * dm.kxb * 2 * nb <= 2 * dm.N ? ks + 2 * dm.kxb * 2 * nb : nullptr;
         rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid,
-                             ks, n_s);
+                             ks, n_s, ko, (uint32_t*)(sh.need + 130));
         if (chg)
           rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
-                               ka, n_a);
+                               ka, n_a, nullptr, nullptr);
         __syncthreads();
         for (int i = tid; i < (chg ? 128 : 64); i += BS) sh.need[i] = 0;
 #pragma unroll
@@ -2022,7 +2045,7 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       }
       __syncthreads();
       rebuild_rows<BS, GW>(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
-                           ka, (int)sh.need[129]);
+                           ka, (int)sh.need[129], nullptr, nullptr);
       __syncthreads();
       for (int i = 64 + tid; i < 128; i += BS) sh.need[i] = 0;
       float m_atpe = 0.0f, m_tear = 0.0f;
