@@ -648,4 +648,142 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// The same expm by the two lanes of a pair (lanes 2m, 2m+1 of a quad): lane
+// h = lane & 1 owns rows 3h..3h+2.  Row k of a product's right operand comes
+// from lane k / 3 of the pair by one DPP quad permutation ([0,0,2,2] or
+// [1,1,3,3]; the owner reads its own value through it), so every row again
+// sees expm_ref's operations in expm_ref's order and the result is bit for
+// bit expm_ref's.  Against the quad form a key costs 2 lanes instead of 4
+// (the quad's fourth lane repeats the third's rows) and a product 72 DPP
+// moves and 108 FMAs per lane instead of 72 and 72: half as many
+// sub-rounds, each about a quarter longer, for the price of 36 VGPRs per
+// matrix.  Both lanes of the pair must be active.
+// ---------------------------------------------------------------------------
+template <int H>
+__device__ __forceinline__ double pair_bcast_d(double v) {
+  constexpr int C = H == 0 ? 0xA0 : 0xF5;  // quad_perm [0,0,2,2] / [1,1,3,3]
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), C, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), C, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int K>
+__device__ __forceinline__ void pair_fma_row(const double A[3][6], const double B[3][6], double acc[3][6]) {
+  double b[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) b[j] = pair_bcast_d<K / 3>(B[K % 3][j]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
+  __builtin_amdgcn_sched_barrier(0);
+}
+// C = A B for the lane's three rows; C must not alias A or B.
+__device__ __forceinline__ void pair_matmul6(const double A[3][6], const double B[3][6], double C[3][6]) {
+  pair_fma_row<0>(A, B, C);
+  pair_fma_row<1>(A, B, C);
+  pair_fma_row<2>(A, B, C);
+  pair_fma_row<3>(A, B, C);
+  pair_fma_row<4>(A, B, C);
+  pair_fma_row<5>(A, B, C);
+}
+
+// A <- expm(A) for a 6x6 matrix held as row triples by a pair; h = lane & 1.
+__device__ inline void expm_pair6(double A[3][6], int h) {
+  double nrm = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rs += fabs(A[i][j]);
+    nrm = fmax(nrm, rs);
+  }
+  nrm = fmax(pair_bcast_d<0>(nrm), pair_bcast_d<1>(nrm));
+  int s = (int)ceilf(log2f((float)nrm / 0.5f + 1e-10f));
+  s = s < 0 ? 0 : s;
+  const double sc = ldexp(1.0, -s);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= sc;
+  const double b0 = 1.0, b1 = 1.0 / 2.0, b2 = 1.0 / 9.0, b3 = 1.0 / 72.0, b4 = 1.0 / 1008.0,
+               b5 = 1.0 / 30240.0, b6 = 1.0 / 1209600.0;
+  double A2[3][6], A4[3][6], A6[3][6];
+  pair_matmul6(A, A, A2);
+  pair_matmul6(A2, A2, A4);
+  pair_matmul6(A4, A2, A6);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double id = (3 * h + i == j) ? 1.0 : 0.0;
+      const double v = b0 * id + b2 * A2[i][j] + b4 * A4[i][j] + b6 * A6[i][j];
+      const double w = b1 * id + b3 * A2[i][j] + b5 * A4[i][j];
+      A6[i][j] = v;
+      A4[i][j] = w;
+    }
+  double (&V)[3][6] = A6;
+  double (&W)[3][6] = A4;
+  pair_matmul6(A, W, A2);  // U
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      W[i][j] = V[i][j] - A2[i][j];
+      A[i][j] = V[i][j] + A2[i][j];
+    }
+  // Gauss-Jordan as in expm_quad6: every lane scales the broadcast pivot
+  // row itself, the owner keeps the scaled row
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double pw[6], pa[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      pw[j] = k < 3 ? pair_bcast_d<0>(W[k % 3][j]) : pair_bcast_d<1>(W[k % 3][j]);
+      pa[j] = k < 3 ? pair_bcast_d<0>(A[k % 3][j]) : pair_bcast_d<1>(A[k % 3][j]);
+    }
+    const double inv = 1.0 / pw[k];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { pw[j] *= inv; pa[j] *= inv; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const bool own = 3 * h + i == k;
+      const double f = W[i][k];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double w = fma(-f, pw[j], W[i][j]), a = fma(-f, pa[j], A[i][j]);
+        W[i][j] = own ? pw[j] : w;
+        A[i][j] = own ? pa[j] : a;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
+  int q = 0;
+  for (; q + 2 <= nsq; q += 2) {
+    pair_matmul6(A, A, A2);
+    pair_matmul6(A2, A2, A);
+  }
+  if (q < nsq) {
+    pair_matmul6(A, A, A2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) A[i][j] = A2[i][j];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rs += A[i][j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double v = A[i][j] / rs;
+      A[i][j] = isnan(v) ? 1.0 / 6.0 : v;
+    }
+  }
+}
+
 }  // namespace mf

@@ -231,6 +231,26 @@ __device__ __forceinline__ void xb_quad_finish(double A[2][6], double dt, int ke
   }
 }
 
+// The same for a pair's row triples (h = lane & 1): expm_pair6, rows
+// 3h..3h+2 stored.  Sampling rows only.
+__device__ __forceinline__ void xb_pair_finish(double A[3][6], double dt, int key, float* xbrows, int h) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= dt;
+  expm_pair6(A, h);
+  float* row = xbrows + (int64_t)key * 48;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float* r = row + (3 * h + i) * 8;
+    double c = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { c += A[i][j]; r[j] = (float)c; }
+    r[6] = 0.0f;
+    r[7] = 0.0f;
+  }
+}
+
 // One mean-field-blended XB key's entry computed by the four lanes of a
 // quad: the populations' generators accumulated in order, as
 // xb_generator<double, true> does; l = lane & 3.
@@ -279,14 +299,18 @@ __device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
 //     at dt = 1 ms, the others 1-11);
 //  2. (with ``order``) a counting sort of the keys by s, largest first,
 //     through the 20 counters of ``hist``;
-//  3. the quads take the keys 16 at a time (a sub-round of one wave, whose
-//     quads square in lockstep, so it lasts as long as its largest s) and
-//     write the rows over the parked rates.  Sub-round j of the sorted list
-//     goes to wave j mod nw, in reverse order on every other pass (a snake),
-//     so the waves' sums of sub-round costs even out.
+//  3. groups of LPK lanes (pairs, expm_pair6, or quads, expm_quad6) take
+//     the keys 64 / LPK at a time (a sub-round of one wave, whose groups
+//     square in lockstep, so it lasts as long as its largest s) and write
+//     the rows over the parked rates.  Sub-round j of the sorted list goes
+//     to wave j mod nw, in reverse order on every other pass (a snake), so
+//     the waves' sums of sub-round costs even out.  Pairs need 36 VGPRs per
+//     matrix against the quads' 24: they are for blocks of up to 512
+//     threads (256 VGPRs per lane).
 // Each key's result depends only on its own rates, not on the order.  All
 // the workgroup's threads call it; ``hist`` must be zero on entry and is
 // zero again on return.
+template <int LPK>
 __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, double d, float dt,
                                                       const float* centers, int nb, int kxb, bool absorbing,
                                                       float* xbrows, float* absrows, int tid, int nthreads,
@@ -354,22 +378,31 @@ __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, 
     if (tid < 20) hist[tid] = 0u;
     __syncthreads();
   }
-  // 3. the quads, sub-round by sub-round in snake order
-  const int rounds = (n + 15) >> 4;
+  // 3. the lane groups, sub-round by sub-round in snake order
+  constexpr int KPW = 64 / LPK;
+  const int rounds = (n + KPW - 1) / KPW;
   for (int pass = 0;; ++pass) {
     const int j = pass * nw + ((pass & 1) ? nw - 1 - wave : wave);
     if (j >= rounds) break;
-    const int pos = j * 16 + (lane >> 2);
+    const int pos = j * KPW + lane / LPK;
     if (pos < n) {
       const int kk = order ? order[pos] : (list ? list[pos] : pos);
       const double* slot = (const double*)(xbrows + (int64_t)kk * 48);
       XbRates<double> rk;
       rk.r01 = slot[0]; rk.r04 = slot[1]; rk.r05 = slot[2]; rk.r10 = slot[3]; rk.r12 = slot[4]; rk.r21 = slot[5];
       rk.r23 = slot[6]; rk.r32 = slot[7]; rk.r34 = slot[8]; rk.r40 = slot[9]; rk.r50 = slot[10];
-      double A[2][6];
+      if constexpr (LPK == 2) {
+        const int h = lane & 1;
+        double A[3][6];
 #pragma unroll
-      for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, absorbing, A[i2]);
-      xb_quad_finish(A, (double)dt, kk, absorbing, xbrows, absrows, l);
+        for (int i2 = 0; i2 < 3; ++i2) xb_gen_row<double>(rk, 3 * h + i2, false, A[i2]);
+        xb_pair_finish(A, (double)dt, kk, xbrows, h);
+      } else {
+        double A[2][6];
+#pragma unroll
+        for (int i2 = 0; i2 < 2; ++i2) xb_gen_row<double>(rk, 2 * rp + i2, false, A[i2]);
+        xb_quad_finish(A, (double)dt, kk, false, xbrows, absrows, l);
+      }
     }
   }
 }
@@ -383,8 +416,8 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
                                                            bool absorbing, float* xbrows, float* absrows,
                                                            int tid, int nthreads, const unsigned short* list,
                                                            int count, unsigned short* order, uint32_t* hist) {
-  xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, nthreads, list, count,
-                        order, hist);
+  xb_table_rebuild_body<(BS <= 512 ? 2 : 4)>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid,
+                                             nthreads, list, count, order, hist);
 }
 
 // The step kernel's rebuild: a call for the LDS-resident variants, inline
@@ -401,8 +434,8 @@ __device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d,
                                              int tid, const unsigned short* list, int count,
                                              unsigned short* order, uint32_t* hist) {
   if constexpr (GW > 0)
-    xb_table_rebuild_body(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count, order,
-                          hist);
+    xb_table_rebuild_body<(BS <= 512 ? 2 : 4)>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid,
+                                               BS, list, count, order, hist);
   else
     xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count, order,
                          hist);
@@ -835,10 +868,45 @@ __device__ __forceinline__ void xb_force(const Springs& sp, T xd, T d, T th, T& 
   }
 }
 
-// Spacing derivatives of a bound XB, for the bordered system of dynamic
-// lattice spacing (solver.py:552-656, forces.py:649-749): f_d = d f / d d,
-// the radial force g (forces.py:708-709) and g_d = d g / d d.  With
-// E the head's energy, f = dE/dxd and g = dE/dd, so d g / d xd = f_d.
+// A bound XB's axial force and stiffness (as xb_force) together with its
+// spacing derivatives, for the bordered system of dynamic lattice spacing
+// (solver.py:552-656, forces.py:649-749): f_d = d f / d d, the radial force
+// g (forces.py:708-709) and g_d = d g / d d.  With E the head's energy,
+// f = dE/dxd and g = dE/dd, so d g / d xd = f_d.  The forces f and g enter
+// the residuals and are formed in T from one reciprocal of r; the three
+// derivatives only steer the Newton and CG iterations and are formed in
+// float, like the stiffness.
+template <typename T>
+__device__ __forceinline__ void xb_force_radial(const Springs& sp, T xd, T d, T th, T& f, float& kf, float& f_d,
+                                                T& g, float& g_d) {
+  if constexpr (!std::is_same<T, double>::value) {
+    // reference mode: the reference's float expressions
+    xb_force<T>(sp, xd, d, th, f, kf);
+    const T r = sqrt(xd * xd + d * d);
+    const T rs = r > T(1e-10) ? r : T(1e-10);
+    const T r3 = rs * rs * rs, r4 = r3 * rs;
+    const T dth = th - T(sp.cr);
+    f_d = (float)(T(sp.gk) * T(sp.gr) * xd * d / r3 - T(sp.ck) * (xd * d + dth * (xd * xd - d * d)) / r4);
+    g = T(sp.gk) * (rs - T(sp.gr)) * (d / rs) + (T(1) / rs) * T(sp.ck) * dth * (xd / rs);
+    g_d = (float)(T(sp.gk) * (T(1) - T(sp.gr) * xd * xd / r3) + T(sp.ck) * xd * (xd - T(2) * d * dth) / r4);
+    return;
+  }
+  const T r = sqrt(xd * xd + d * d);
+  const T rs = r > T(1e-10) ? r : T(1e-10);
+  const T inv = T(1) / rs;
+  const T dth = th - T(sp.cr);
+  f = (T(sp.gk) * (r - T(sp.gr)) * xd - inv * T(sp.ck) * dth * d) * inv;
+  g = T(sp.gk) * (rs - T(sp.gr)) * (d * inv) + inv * T(sp.ck) * dth * (xd * inv);
+  const float dfl = (float)d, xf = (float)xd, dthf = (float)th - sp.cr;
+  const float i1 = __builtin_amdgcn_rcpf((float)rs), i2 = i1 * i1, i3 = i2 * i1, i4 = i2 * i2;
+  kf = sp.gk * (1.0f - sp.gr * dfl * dfl * i3) + sp.ck * dfl * (dfl + 2.0f * xf * dthf) * i4;
+  f_d = sp.gk * sp.gr * xf * dfl * i3 - sp.ck * (xf * dfl + dthf * (xf * xf - dfl * dfl)) * i4;
+  g_d = sp.gk * (1.0f - sp.gr * xf * xf * i3) + sp.ck * xf * (xf - 2.0f * dfl * dthf) * i4;
+}
+
+// The same spacing derivatives on their own, for the workspace variants
+// (those keep this split form: the merged one, though it does less work,
+// moves their spills into the CG loop and costs them about 5 %).
 template <typename T>
 __device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, T th, float& f_d, T& g, float& g_d) {
   const T r = sqrt(xd * xd + d * d);
@@ -850,9 +918,38 @@ __device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, T th, fl
   g_d = (float)(T(sp.gk) * (T(1) - T(sp.gr) * xd * xd / r3) + T(sp.ck) * xd * (xd - T(2) * d * dth) / r4);
 }
 
-// Titin on the tip crown for the bordered system: the radial force tr
-// (N_TITIN F d / L), its spacing derivative tr_d, and tt_d = d(axial titin
-// force)/d d, which is also d(radial residual)/d x_tip.
+// Titin axial force on the tip crown and its derivative with respect to the
+// tip position (forces.py:136-157).  RAD: also, for the bordered system, the
+// radial force tr (N_TITIN F d / L), its spacing derivative tr_d, and
+// tt_d = d(axial titin force)/d d, which is also d(radial residual)/d x_tip.
+template <typename T, bool RAD = false>
+__device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt, T& energy,
+                                      T* tr = nullptr, float* tr_d = nullptr, float* tt_d = nullptr) {
+  const T ad = z - tip;
+  const T Lq = sqrt(ad * ad + d * d);
+  const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
+  const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
+  const T E = exp(argc);
+  // stored energy a/b (exp(b (L - rest)) - 1) (metrics_fn.py), from the
+  // unclamped exponent
+  energy = T(prm[P_TITIN_A]) / T(prm[P_TITIN_B]) * ((arg == argc ? E : exp(arg)) - T(1));
+  T F = T(prm[P_TITIN_A]) * E;
+  F = F > T(0) ? F : T(0);
+  const T cs = Lq > T(0) ? ad / Lq : T(0);
+  tt = T(N_TITIN) * F * cs;
+  const T Fp = (arg > T(-100) && arg < T(100)) ? T(prm[P_TITIN_B]) * F : T(0);
+  ktt = (float)(-T(N_TITIN) * (Fp * ad * ad / (Lq * Lq) + F * d * d / (Lq * Lq * Lq)));
+  if constexpr (RAD) {
+    const T Ls = Lq > T(1e-10) ? Lq : T(1e-10);
+    const T il = T(1) / Ls, il2 = il * il, il3 = il2 * il;
+    *tr = T(N_TITIN) * F * (d * il);
+    *tr_d = (float)(T(N_TITIN) * (Fp * d * d * il2 + F * ad * ad * il3));
+    *tt_d = (float)(T(N_TITIN) * ad * d * (Fp * il2 - F * il3));
+  }
+}
+
+// The titin radial terms on their own (as titin<T, true> forms them), for
+// the workspace variants.
 template <typename T>
 __device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T d, T& tr, float& tr_d,
                                              float& tt_d) {
@@ -868,26 +965,6 @@ __device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T
   const T L2 = Ls * Ls, L3 = L2 * Ls;
   tr_d = (float)(T(N_TITIN) * (bF * d * d / L2 + F * ad * ad / L3));
   tt_d = (float)(T(N_TITIN) * ad * d * (bF / L2 - F / L3));
-}
-
-// Titin axial force on the tip crown and its derivative with respect to the
-// tip position (forces.py:136-157).
-template <typename T>
-__device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt, T& energy) {
-  const T ad = z - tip;
-  const T Lq = sqrt(ad * ad + d * d);
-  const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
-  const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
-  const T E = exp(argc);
-  // stored energy a/b (exp(b (L - rest)) - 1) (metrics_fn.py), from the
-  // unclamped exponent
-  energy = T(prm[P_TITIN_A]) / T(prm[P_TITIN_B]) * ((arg == argc ? E : exp(arg)) - T(1));
-  T F = T(prm[P_TITIN_A]) * E;
-  F = F > T(0) ? F : T(0);
-  const T cs = Lq > T(0) ? ad / Lq : T(0);
-  tt = T(N_TITIN) * F * cs;
-  const T Fp = (arg > T(-100) && arg < T(100)) ? T(prm[P_TITIN_B]) * F : T(0);
-  ktt = (float)(-T(N_TITIN) * (Fp * ad * ad / (Lq * Lq) + F * d * d / (Lq * Lq * Lq)));
 }
 
 // A site's list of bound heads, relinked into increasing head order.  The
@@ -1004,14 +1081,26 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       const int crown = j / dm.nx;
       const T xd = T(posd<REF64>(sh, site)) - T(posd<REF64>(sh, crown));
       const Springs sp = springs(prm, st);
-      const T th = dyn ? xb_angle<T, true>(d, xd) : xb_angle<T, false>(d, xd);
-      xb_force<T>(sp, xd, d, th, fj, kj);
-      if (dyn) {
-        T g;
-        float gd;
-        xb_radial<T>(sp, xd, d, th, fd, g, gd);
-        sg += g;
-        sgd += T(gd);
+      if constexpr (GW == 0) {
+        if (dyn) {
+          T g;
+          float gd;
+          xb_force_radial<T>(sp, xd, d, xb_angle<T, true>(d, xd), fj, kj, fd, g, gd);
+          sg += g;
+          sgd += T(gd);
+        } else {
+          xb_force<T>(sp, xd, d, xb_angle<T, false>(d, xd), fj, kj);
+        }
+      } else {
+        const T th = dyn ? xb_angle<T, true>(d, xd) : xb_angle<T, false>(d, xd);
+        xb_force<T>(sp, xd, d, th, fj, kj);
+        if (dyn) {
+          T g;
+          float gd;
+          xb_radial<T>(sp, xd, d, th, fd, g, gd);
+          sg += g;
+          sgd += T(gd);
+        }
       }
     }
     if constexpr (REF64) sh.fx64[j] = (double)fj; else sh.fx[j] = (float)fj;
@@ -1059,7 +1148,10 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
       } else {
         float kt;
         T en;
-        titin<T>(prm, x0, zl, d, s_hi, kt, en);
+        T tr = T(0);
+        float trd = 0.0f, ttd = 0.0f;
+        if constexpr (GW == 0) titin<T, DYN>(prm, x0, zl, d, s_hi, kt, en, &tr, &trd, &ttd);
+        else titin<T>(prm, x0, zl, d, s_hi, kt, en);
         sh.ktip[sl.fil[s]] = kt;
         if constexpr (REF64) {
           // the tip's titin energy for the metrics: the step's first
@@ -1070,9 +1162,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
           if (first) fv[F_EO] = (double)en;
         }
         if (dyn) {
-          T tr;
-          float trd, ttd;
-          titin_radial<T>(prm, x0, zl, d, tr, trd, ttd);
+          if constexpr (GW > 0) titin_radial<T>(prm, x0, zl, d, tr, trd, ttd);
           sg += tr;
           sgd += T(trd);
           cs += ttd;
