@@ -216,50 +216,67 @@ template <typename T>
 struct XbRates { T r01, r04, r05, r10, r12, r21, r23, r32, r34, r40, r50; };
 
 // The parts of the XB rates that depend on neither the head's offset nor
-// the spacing: kT and the calcium-dependent SRX rate r50.
+// the spacing: kT (and 1/kT), the calcium-dependent SRX rate r50, and the
+// constant factors e^(U_loose - U_DRX) and e^(U_tight_2 - U_tight_1) of the
+// reverse rates r10 and r32.
 template <typename T>
-struct XbPre { T kt, r50; };
+struct XbPre { T kt, ikt, r50, e10, e32; };
 
 template <typename T>
 __device__ inline XbPre<T> xb_pre(const float* prm, T pCa) {
   XbPre<T> p;
   p.kt = kT_pNnm<T>(T(prm[P_TEMP]));
+  p.ikt = T(1) / p.kt;
   const T ca = ca_conc<T>(pCa), b = prm[P_SRX_B];
   const T cab = pow(ca, b);
   p.r50 = T(prm[P_SRX_K0]) + ((T(prm[P_SRX_KMAX]) - T(prm[P_SRX_K0])) * cab) / (pow(T(prm[P_SRX_CA50]), b) + cab);
+  p.e10 = exp(T(prm[P_U_LOOSE]) - T(prm[P_U_DRX]));
+  p.e32 = exp(T(prm[P_U_T2]) - T(prm[P_U_T1]));
   return p;
 }
 
+// The reference forms the reverse rates on a log scale:
+// r10 = exp(log(r01 + 0.005) - (U_DRX - U_loose)) (rate_functions.py:384-387),
+// r21 = exp(log(r12) - (U_loose - U_t1)) (:464-466),
+// r32 = exp(min(log(r23 + 1e-30) + (U_t2 - U_t1), log(cap))) (:559-560).
+// Here the same products are formed directly, with
+// e^(U_loose - U_DRX) e^(E_w) = e10 / e^(-E_w) and e^(U_t2 - U_t1) = e32
+// (the strong-state energies cancel): five exponentials and no logarithm
+// instead of seven and three.  The results agree to rounding (relative
+// 1e-15); the caps, the NaN rules and the underflow ends are those of the
+// log-scale forms.
 template <typename T>
 __device__ inline XbRates<T> xb_rates(const float* prm, const XbPre<T>& pre, T d, T x, T perm) {
   const T cap = T(RATE_CAP);
   const T r = sqrt(x * x + d * d);
   const T th = atan2(d, x);
-  const T kt = pre.kt;
+  const T ikt = pre.ikt;
   const T dgw = r - T(prm[P_G_REST_W]), dcw = th - T(prm[P_C_REST_W]);
   const T dgs = r - T(prm[P_G_REST_S]), dcs = th - T(prm[P_C_REST_S]);
-  const T Ew = (T(0.5) * T(prm[P_G_K_W]) * (dgw * dgw) + T(0.5) * T(prm[P_C_K_W]) * (dcw * dcw)) / kt;
-  const T Es = (T(0.5) * T(prm[P_G_K_S]) * (dgs * dgs) + T(0.5) * T(prm[P_C_K_S]) * (dcs * dcs)) / kt;
+  const T Ew = (T(0.5) * T(prm[P_G_K_W]) * (dgw * dgw) + T(0.5) * T(prm[P_C_K_W]) * (dcw * dcw)) * ikt;
+  const T Es = (T(0.5) * T(prm[P_G_K_S]) * (dgs * dgs) + T(0.5) * T(prm[P_C_K_S]) * (dcs * dcs)) * ikt;
   const T Ediff = Ew - Es;
-  const T Udrx = prm[P_U_DRX];
   const T Uloose = T(prm[P_U_LOOSE]) + Ew;
   const T Ut1 = T(prm[P_U_T1]) + Es;
-  const T Ut2 = T(prm[P_U_T2]) + Es;
   const T fs = T(prm[P_G_K_S]) * dgs + (T(1) / r) * T(prm[P_C_K_S]) * dcs;
   XbRates<T> k;
-  T r01 = perm * T(prm[P_R01]) * exp(-Ew);
+  const T emw = exp(-Ew);
+  T r01 = perm * T(prm[P_R01]) * emw;
   if (isnan(r01)) r01 = T(0);
-  T r10 = fmin(exp(log(r01 + T(0.005)) - (Udrx - Uloose)), cap);
+  // e^(-E_w) = 0 (E_w > 745) makes the factor infinite and r10 = cap, where
+  // the log-scale form's exponential overflows
+  T r10 = fmin((r01 + T(0.005)) * (pre.e10 / emw), cap);
   if (isnan(r10)) r10 = cap;
   T r12 = T(prm[P_R12]) * exp(fmin(Ediff, T(30)) / T(2));
   if (isnan(r12)) r12 = T(0);
   k.r01 = r01;
   k.r10 = r10;
   k.r12 = r12;
-  k.r21 = fmin(exp(log(r12) - (Uloose - Ut1)), cap);
-  k.r23 = fmin(T(prm[P_R23]) * exp(-fs * T(prm[P_D23]) / kt), cap);
-  k.r32 = exp(fmin(log(k.r23 + T(1e-30)) + (Ut2 - Ut1), log(cap)));
-  k.r34 = fmin(T(prm[P_R34]) * exp(fs * T(prm[P_D34]) / kt), cap);
+  // r12 = 0 (underflow) has log -inf and r21 = 0 in the log-scale form
+  k.r21 = r12 > T(0) ? fmin(r12 * exp(Ut1 - Uloose), cap) : T(0);
+  k.r23 = fmin(T(prm[P_R23]) * exp(-fs * T(prm[P_D23]) * ikt), cap);
+  k.r32 = fmin((k.r23 + T(1e-30)) * pre.e32, cap);
+  k.r34 = fmin(T(prm[P_R34]) * exp(fs * T(prm[P_D34]) * ikt), cap);
   k.r40 = prm[P_R40];
   k.r04 = prm[P_R04];
   k.r05 = prm[P_R05];

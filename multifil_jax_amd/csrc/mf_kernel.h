@@ -120,12 +120,35 @@ __device__ __forceinline__ void tm_table_entry(const Pops& P, int set, double pC
   }
 }
 
+// The XB rates' offset- and spacing-free parts (XbPre) of parameter set k
+// at pCa: from the cache (five doubles per set) when there is one, else
+// computed (three powers and two exponentials).
+__device__ __forceinline__ XbPre<double> xb_pre_of(const Pops& P, const double* cache, double pCa, int k) {
+  if (cache) {
+    const double* c = cache + 5 * k;
+    XbPre<double> p;
+    p.kt = c[0]; p.ikt = c[1]; p.r50 = c[2]; p.e10 = c[3]; p.e32 = c[4];
+    return p;
+  }
+  return xb_pre<double>(P.prm + k * MF_N_PARAMS, pCa);
+}
+// Fill the cache for every parameter set at pCa (threads 0..K-1; the caller
+// puts a barrier between this and the readers).
+__device__ __forceinline__ void fill_xb_pre(const Pops& P, double pCa, LDS double* cache, int tid) {
+  if (tid < P.K) {
+    const XbPre<double> p = xb_pre<double>(P.prm + tid * MF_N_PARAMS, pCa);
+    LDS double* c = cache + 5 * tid;
+    c[0] = p.kt; c[1] = p.ikt; c[2] = p.r50; c[3] = p.e10; c[4] = p.e32;
+  }
+}
+
 // The absorbing-chain entry of XB key (set * 2 nb + perm * nb + bin) at
 // spacing d: P[3][4], P[2][0], P[3][0] of the chain with states 0 and 4
 // absorbing (transitions.py:1029-1078), from its 3x3 transient block
-// (absorbing_probs); mean-field populations blend the block.
-__device__ __forceinline__ void absorbing_entry(const Pops& P, double pCa, double d, double dt, double center,
-                                                int key, int nb, float* absrows) {
+// (absorbing_probs); mean-field populations blend the block.  ``pre``: the
+// XbPre cache at pCa, or null.
+__device__ __forceinline__ void absorbing_entry(const Pops& P, const double* pre, double pCa, double d, double dt,
+                                                double center, int key, int nb, float* absrows) {
   const int set = key / (2 * nb), k2 = key % (2 * nb);
   const double perm = (double)(k2 / nb);
   double a[3][3], e10 = 0.0, e34 = 0.0;
@@ -135,11 +158,13 @@ __device__ __forceinline__ void absorbing_entry(const Pops& P, double pCa, doubl
 #pragma unroll
       for (int j = 0; j < 3; ++j) a[i][j] = 0.0;
     for (int k = 0; k < P.K; ++k) {
-      const XbRates<double> r = xb_rates<double>(P.prm + k * MF_N_PARAMS, pCa, d, center, perm);
+      const XbRates<double> r =
+          xb_rates<double>(P.prm + k * MF_N_PARAMS, xb_pre_of(P, pre, pCa, k), d, center, perm);
       absorbing_block<double, true>(r, dt, (double)P.frac[k], a, e10, e34);
     }
   } else {
-    const XbRates<double> r = xb_rates<double>(P.prm + set * MF_N_PARAMS, pCa, d, center, perm);
+    const XbRates<double> r =
+        xb_rates<double>(P.prm + set * MF_N_PARAMS, xb_pre_of(P, pre, pCa, set), d, center, perm);
     absorbing_block<double, false>(r, dt, 1.0, a, e10, e34);
   }
   double o[3];
@@ -161,7 +186,7 @@ __device__ __forceinline__ void xb_table_entry(const Pops& P, double pCa, double
   const int set = key / (2 * nb), k2 = key % (2 * nb);
   const double perm = (double)(k2 / nb);
   if (absorbing) {
-    absorbing_entry(P, pCa, d, dt, center, key, nb, absrows);
+    absorbing_entry(P, nullptr, pCa, d, dt, center, key, nb, absrows);
     return;
   }
   double Q[6][6];
@@ -254,9 +279,9 @@ __device__ __forceinline__ void xb_pair_finish(double A[3][6], double dt, int ke
 // One mean-field-blended XB key's entry computed by the four lanes of a
 // quad: the populations' generators accumulated in order, as
 // xb_generator<double, true> does; l = lane & 3.
-__device__ __forceinline__ void xb_blend_entry_quad(const Pops& P, double pCa, double d, double dt, double center,
-                                                    int key, int nb, bool absorbing, float* xbrows, float* absrows,
-                                                    int l) {
+__device__ __forceinline__ void xb_blend_entry_quad(const Pops& P, const double* pre, double pCa, double d,
+                                                    double dt, double center, int key, int nb, bool absorbing,
+                                                    float* xbrows, float* absrows, int l) {
   const double perm = (double)((key % (2 * nb)) / nb);
   const int rp = l < 2 ? l : 2;
   double A[2][6];
@@ -265,7 +290,8 @@ __device__ __forceinline__ void xb_blend_entry_quad(const Pops& P, double pCa, d
 #pragma unroll
     for (int j = 0; j < 6; ++j) A[i][j] = 0.0;
   for (int k = 0; k < P.K; ++k) {
-    const XbRates<double> r = xb_rates<double>(P.prm + k * MF_N_PARAMS, pCa, d, center, perm);
+    const XbRates<double> r =
+        xb_rates<double>(P.prm + k * MF_N_PARAMS, xb_pre_of(P, pre, pCa, k), d, center, perm);
     const double w = (double)P.frac[k];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -276,16 +302,6 @@ __device__ __forceinline__ void xb_blend_entry_quad(const Pops& P, double pCa, d
     }
   }
   xb_quad_finish(A, dt, key, absorbing, xbrows, absrows, l);
-}
-
-template <typename T>
-__device__ __forceinline__ XbRates<T> shfl_rates(const XbRates<T>& r, int src) {
-  XbRates<T> o;
-  o.r01 = __shfl(r.r01, src); o.r04 = __shfl(r.r04, src); o.r05 = __shfl(r.r05, src);
-  o.r10 = __shfl(r.r10, src); o.r12 = __shfl(r.r12, src); o.r21 = __shfl(r.r21, src);
-  o.r23 = __shfl(r.r23, src); o.r32 = __shfl(r.r32, src); o.r34 = __shfl(r.r34, src);
-  o.r40 = __shfl(r.r40, src); o.r50 = __shfl(r.r50, src);
-  return o;
 }
 
 // The kxb * 2 nb XB (or absorbing) entries of one instance at spacing d.
@@ -315,39 +331,33 @@ __device__ __forceinline__ void xb_table_rebuild_body(const Pops& P, float pCa, 
                                                       const float* centers, int nb, int kxb, bool absorbing,
                                                       float* xbrows, float* absrows, int tid, int nthreads,
                                                       const unsigned short* list, int count,
-                                                      unsigned short* order, uint32_t* hist) {
+                                                      unsigned short* order, uint32_t* hist, const double* pre) {
   const int l = tid & 3;
   const int n = list ? count : kxb * 2 * nb;
   if (absorbing) {
     // one lane per key: a 3x3 exponential (absorbing_entry)
     for (int i = tid; i < n; i += nthreads) {
       const int key = list ? list[i] : i;
-      absorbing_entry(P, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absrows);
+      absorbing_entry(P, pre, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absrows);
     }
     return;
   }
   if (P.blend_xb) {
     for (int i = tid >> 2; i < n; i += nthreads >> 2) {
       const int key = list ? list[i] : i;
-      xb_blend_entry_quad(P, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absorbing, xbrows,
-                          absrows, l);
+      xb_blend_entry_quad(P, pre, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absorbing,
+                          xbrows, absrows, l);
     }
     return;
   }
   const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6, rp = l < 2 ? l : 2;
-  // lane s < kxb holds parameter set s's key-independent terms
-  const XbPre<double> pre = xb_pre<double>(P.prm + (lane < kxb ? lane : kxb - 1) * MF_N_PARAMS, (double)pCa);
   // 1. rates and s per key, parked in the key's slot (11 doubles, then s)
-  for (int b0 = 0; b0 < n; b0 += nthreads) {
-    const int i = b0 + tid;
-    const int key = i < n ? (list ? list[i] : i) : 0;
+  for (int i = tid; i < n; i += nthreads) {
+    const int key = list ? list[i] : i;
     const int set = key / (2 * nb), k2 = key % (2 * nb);
-    XbPre<double> ps;
-    ps.kt = __shfl(pre.kt, set);
-    ps.r50 = __shfl(pre.r50, set);
-    if (i < n) {
-      const XbRates<double> r =
-          xb_rates<double>(P.prm + set * MF_N_PARAMS, ps, d, (double)centers[key % nb], (double)(k2 / nb));
+    {
+      const XbRates<double> r = xb_rates<double>(P.prm + set * MF_N_PARAMS, xb_pre_of(P, pre, (double)pCa, set), d,
+                                                 (double)centers[key % nb], (double)(k2 / nb));
       double* slot = (double*)(xbrows + (int64_t)key * 48);
       slot[0] = r.r01; slot[1] = r.r04; slot[2] = r.r05; slot[3] = r.r10; slot[4] = r.r12; slot[5] = r.r21;
       slot[6] = r.r23; slot[7] = r.r32; slot[8] = r.r34; slot[9] = r.r40; slot[10] = r.r50;
@@ -415,9 +425,10 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
                                                            const float* centers, int nb, int kxb,
                                                            bool absorbing, float* xbrows, float* absrows,
                                                            int tid, int nthreads, const unsigned short* list,
-                                                           int count, unsigned short* order, uint32_t* hist) {
+                                                           int count, unsigned short* order, uint32_t* hist,
+                                                           const double* pre) {
   xb_table_rebuild_body<(BS <= 512 ? 2 : 4)>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid,
-                                             nthreads, list, count, order, hist);
+                                             nthreads, list, count, order, hist, pre);
 }
 
 // The step kernel's rebuild: a call for the LDS-resident variants, inline
@@ -432,13 +443,13 @@ template <int BS, int GW>
 __device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d, float dt, const float* centers,
                                              int nb, int kxb, bool absorbing, float* xbrows, float* absrows,
                                              int tid, const unsigned short* list, int count,
-                                             unsigned short* order, uint32_t* hist) {
+                                             unsigned short* order, uint32_t* hist, const double* pre) {
   if constexpr (GW > 0)
     xb_table_rebuild_body<(BS <= 512 ? 2 : 4)>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid,
-                                               BS, list, count, order, hist);
+                                               BS, list, count, order, hist, pre);
   else
     xb_table_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid, BS, list, count, order,
-                         hist);
+                         hist, pre);
 }
 
 // The keys marked in a 64-word bitmap, in increasing order, into list[]; run
@@ -555,6 +566,7 @@ struct Shared {
   LDS uint32_t* need;    // [2][64] bitmaps of the XB keys a step reads (sampling, absorbing),
                          // then the two compacted key counts, then the rebuild's
                          // [20] counters and [20] cursors of its sort by squaring count
+  LDS double* xbpre;     // [MF_MAX_POPS][5] XbPre of each parameter set at the rebuild's pCa
 };
 
 // LDS / workspace carve-up.  One routine computes the byte offsets (host:
@@ -563,7 +575,7 @@ struct Shared {
 // the workspace slice when GW is set.
 struct LdsOffsets {
   int64_t x, pv, fx, kx, xsite, xnext, xkey, xslot, org, cbn, xst, shead, tbound, tm, fsites, fcount, tprev, tnext, tmcum, need,
-      edges, prm, red, wpart, fil, ktip, cnt, total, ws_total;
+      edges, prm, red, wpart, fil, ktip, cnt, xbpre, total, ws_total;
 };
 template <int BS, bool REF64, int GW = 0>
 __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
@@ -605,6 +617,7 @@ __host__ __device__ inline LdsOffsets lds_offsets(const Dims& d) {
   L.ktip = take(4LL * d.Tk);
   L.cnt = take(4 * NC * NW);
   L.need = take(4 * (2 * 64 + 2 + 40));
+  L.xbpre = take(8 * 5 * MF_MAX_POPS);
   L.total = o;
   L.ws_total = w;
   return L;
@@ -655,6 +668,7 @@ __device__ __forceinline__ Shared<REF64, GW> carve(LDS char* base, char* ws, con
   t.ktip = (LDS float*)(base + o.ktip);
   t.cnt = (LDS int*)(base + o.cnt);
   t.need = (LDS uint32_t*)(base + o.need);
+  t.xbpre = (LDS double*)(base + o.xbpre);
   return t;
 }
 
@@ -1787,6 +1801,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
   // fixed-spacing kernels; the host runs traces on these variants)
   const bool trace = DYN && (L.launch_flags & MF_LAUNCH_REBUILD) != 0;
   float tab_pca = d_p, tab_ls = d_l;
+  // pCa of the XbPre cache (NaN: not filled yet)
+  float pre_pca = __builtin_nanf("");
   const bool solve_only = (L.launch_flags & MF_LAUNCH_SOLVE_ONLY) != 0;
   for (int step = init ? -1 : 0; step < n_steps; ++step) {
     // The thread index is made opaque once per step: otherwise the compiler
@@ -1978,6 +1994,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         // one, simulation.py:691-693), and for a changed fixed-spacing
         // instance the marked absorbing rows too, then the prefetch
         const double d_kin = (DYN && rad.on) ? ls_carry : (double)lsd;
+        if (pCa != pre_pca) {  // (barriers follow before the rebuild reads it)
+          fill_xb_pre(pops, (double)pCa, sh.xbpre, tid);
+          pre_pca = pCa;
+        }
         // the marked keys as compact lists (in the CG buffer, idle during
         // the kinetics), so the busy quads are contiguous
         // (one list per kind, 2 bytes per key; the CG buffer holds 2 N; a
@@ -2011,10 +2031,10 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
         // when the buffer holds a third
         unsigned short* ko = 3 * dm.kxb * 2 * nb <= 2 * dm.N ? ks + 2 * dm.kxb * 2 * nb : nullptr;
         rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, false, xbtab, abstab, tid,
-                             ks, n_s, ko, (uint32_t*)(sh.need + 130));
+                             ks, n_s, ko, (uint32_t*)(sh.need + 130), (const double*)sh.xbpre);
         if (chg)
           rebuild_rows<BS, GW>(pops, pCa, d_kin, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
-                               ka, n_a, nullptr, nullptr);
+                               ka, n_a, nullptr, nullptr, (const double*)sh.xbpre);
         __syncthreads();
         for (int i = tid; i < (chg ? 128 : 64); i += BS) sh.need[i] = 0;
 #pragma unroll
@@ -2129,13 +2149,17 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       // the metrics' absorbing chain at the solved spacing (simulation.py:711-719),
       // evaluated on the pre-kinetics keys and states (metrics_fn.py:297-335)
       unsigned short* ka = (unsigned short*)sh.pv;   // the CG buffer is idle after the solve
+      if (pCa != pre_pca) {
+        fill_xb_pre(pops, (double)pCa, sh.xbpre, tid);
+        pre_pca = pCa;
+      }
       if (wave == 0) {
         const int na_ = compact_keys(sh.need + 64, ka, lane);
         if (lane == 0) sh.need[129] = (uint32_t)na_;
       }
       __syncthreads();
       rebuild_rows<BS, GW>(pops, pCa, ls_new, L.cfg.dt, centers, nb, dm.kxb, true, xbtab, abstab, tid,
-                           ka, (int)sh.need[129], nullptr, nullptr);
+                           ka, (int)sh.need[129], nullptr, nullptr, (const double*)sh.xbpre);
       __syncthreads();
       for (int i = 64 + tid; i < 128; i += BS) sh.need[i] = 0;
       float m_atpe = 0.0f, m_tear = 0.0f;
