@@ -23,6 +23,38 @@ constexpr float N_TITIN = 6.0f;
 constexpr float XB_REACH = 13.0f;
 constexpr int WAVE = 64;
 
+// e^x in double precision for the step kernel's loops: Cody-Waite reduction
+// x = n ln2 + r with |r| <= ln2 / 2 (ln 2 split as 6.93147180369123816490e-01
+// + 1.90821492927058770002e-10, the fdlibm split, so that n ln2_hi is exact
+// for |n| < 2^11), the degree-13 Taylor polynomial of e^r by Horner's rule
+// (truncation below 1e-17 relative), then the scaling by 2^n.  Within 2 ulp
+// of e^x (tests/test_exp_f64.py checks it against the C library).  On the
+// device the coefficients are read through a pointer the compiler cannot see
+// through, so in a loop it reloads them (scalar loads) instead of holding
+// fourteen doubles in registers across the loop, as it does with the
+// library exp's constants (and then spills and reloads them every use).
+#define MF_EXP_TAYLOR                                                                             \
+  {1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,     \
+   1.0 / 40320.0,      1.0 / 5040.0,      1.0 / 720.0,      1.0 / 120.0,     1.0 / 24.0,         \
+   1.0 / 6.0,          0.5,               1.0,              1.0}
+static __constant__ double kExpTaylor[14] = MF_EXP_TAYLOR;
+__host__ __device__ inline double exp_f64(double x) {
+  if (!(x <= 709.782712893384)) return x != x ? x : __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  const double n = rint(x * 1.4426950408889634);
+  const double r = fma(-n, 1.90821492927058770002e-10, fma(-n, 6.93147180369123816490e-01, x));
+#ifdef __HIP_DEVICE_COMPILE__
+  const __attribute__((address_space(4))) double* c = (const __attribute__((address_space(4))) double*)kExpTaylor;
+  asm volatile("" : "+s"(c));
+#else
+  static const double c[14] = MF_EXP_TAYLOR;
+#endif
+  double p = c[0];
+#pragma unroll
+  for (int k = 1; k < 14; ++k) p = fma(p, r, c[k]);
+  return ldexp(p, (int)n);
+}
+
 // Per-instance table layout (floats).
 constexpr int TAB_HDR = 16;                 // pCa, d_kin, d_abs, valid, ...
 constexpr int TAB_TM = TAB_HDR;             // ktm x 27 configs x 4 rows x 4 cumsums

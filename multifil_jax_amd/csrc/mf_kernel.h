@@ -836,6 +836,16 @@ __device__ __forceinline__ float posf(const Shared<REF64, GW>& sh, int i) {
   if constexpr (REF64) return (float)sh.x64[i]; else return sh.x32[i];
 }
 
+// e^x in the residual's type.  OWN (kernels with three or more node slots
+// per lane, whose register pressure makes the compiler hold and spill the
+// library exp's constants): exp_f64 in double; otherwise the library's exp
+// (the two-slot kernels keep its constants in registers at no cost: the
+// 2x2 fixed-spacing step is 7 % slower with exp_f64).
+template <typename T, bool OWN>
+__device__ __forceinline__ T exp_t(T x) {
+  if constexpr (OWN && std::is_same<T, double>::value) return exp_f64(x); else return exp(x);
+}
+
 // XB spring constants of a state (forces.py:378-394).
 struct Springs { float gk, gr, ck, cr; };
 __device__ __forceinline__ Springs springs(const LDS float* prm, int st) {
@@ -936,17 +946,17 @@ __device__ __forceinline__ void xb_radial(const Springs& sp, T xd, T d, T th, fl
 // tip position (forces.py:136-157).  RAD: also, for the bordered system, the
 // radial force tr (N_TITIN F d / L), its spacing derivative tr_d, and
 // tt_d = d(axial titin force)/d d, which is also d(radial residual)/d x_tip.
-template <typename T, bool RAD = false>
+template <typename T, bool RAD = false, bool OWN = false>
 __device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& tt, float& ktt, T& energy,
                                       T* tr = nullptr, float* tr_d = nullptr, float* tt_d = nullptr) {
   const T ad = z - tip;
   const T Lq = sqrt(ad * ad + d * d);
   const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
   const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
-  const T E = exp(argc);
+  const T E = exp_t<T, OWN>(argc);
   // stored energy a/b (exp(b (L - rest)) - 1) (metrics_fn.py), from the
   // unclamped exponent
-  energy = T(prm[P_TITIN_A]) / T(prm[P_TITIN_B]) * ((arg == argc ? E : exp(arg)) - T(1));
+  energy = T(prm[P_TITIN_A]) / T(prm[P_TITIN_B]) * ((arg == argc ? E : exp_t<T, OWN>(arg)) - T(1));
   T F = T(prm[P_TITIN_A]) * E;
   F = F > T(0) ? F : T(0);
   const T cs = Lq > T(0) ? ad / Lq : T(0);
@@ -964,7 +974,7 @@ __device__ __forceinline__ void titin(const LDS float* prm, T tip, T z, T d, T& 
 
 // The titin radial terms on their own (as titin<T, true> forms them), for
 // the workspace variants.
-template <typename T>
+template <typename T, bool OWN = false>
 __device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T d, T& tr, float& tr_d,
                                              float& tt_d) {
   const T ad = z - tip;
@@ -972,7 +982,7 @@ __device__ __forceinline__ void titin_radial(const LDS float* prm, T tip, T z, T
   const T Ls = Lq > T(1e-10) ? Lq : T(1e-10);
   const T arg = T(prm[P_TITIN_B]) * (Lq - T(prm[P_TITIN_REST]));
   const T argc = arg < T(-100) ? T(-100) : (arg > T(100) ? T(100) : arg);
-  T F = T(prm[P_TITIN_A]) * exp(argc);
+  T F = T(prm[P_TITIN_A]) * exp_t<T, OWN>(argc);
   F = F > T(0) ? F : T(0);
   const T bF = (arg > T(-100) && arg < T(100)) ? T(prm[P_TITIN_B]) * F : T(0);
   tr = T(N_TITIN) * F * (d / Ls);
@@ -1164,8 +1174,8 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
         T en;
         T tr = T(0);
         float trd = 0.0f, ttd = 0.0f;
-        if constexpr (GW == 0) titin<T, DYN>(prm, x0, zl, d, s_hi, kt, en, &tr, &trd, &ttd);
-        else titin<T>(prm, x0, zl, d, s_hi, kt, en);
+        if constexpr (GW == 0) titin<T, DYN, (MS >= 3)>(prm, x0, zl, d, s_hi, kt, en, &tr, &trd, &ttd);
+        else titin<T, false, (MS >= 3)>(prm, x0, zl, d, s_hi, kt, en);
         sh.ktip[sl.fil[s]] = kt;
         if constexpr (REF64) {
           // the tip's titin energy for the metrics: the step's first
@@ -1176,7 +1186,7 @@ __device__ __forceinline__ float residual(const Shared<REF64, GW>& sh, const Dim
           if (first) fv[F_EO] = (double)en;
         }
         if (dyn) {
-          if constexpr (GW > 0) titin_radial<T>(prm, x0, zl, d, tr, trd, ttd);
+          if constexpr (GW > 0) titin_radial<T, (MS >= 3)>(prm, x0, zl, d, tr, trd, ttd);
           sg += tr;
           sgd += T(trd);
           cs += ttd;
@@ -2211,8 +2221,8 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
             } else {
               const double an = (double)zl - x, ao = (double)zl - fv[F_XT];
               const double Ln = sqrt(an * an + ls_new * ls_new), Lo = sqrt(ao * ao + ls_new * ls_new);
-              const double en = (ta / tbp) * (exp(tbp * (Ln - trest)) - 1.0);
-              const double eo = (ta / tbp) * (exp(tbp * (Lo - trest)) - 1.0);
+              const double en = (ta / tbp) * (exp_t<double, (MS >= 3)>(tbp * (Ln - trest)) - 1.0);
+              const double eo = (ta / tbp) * (exp_t<double, (MS >= 3)>(tbp * (Lo - trest)) - 1.0);
               fv[F_TIT] = en;
               fv[F_DTIT] = en - eo;
             }
