@@ -55,6 +55,43 @@ __host__ __device__ inline double exp_f64(double x) {
   return ldexp(p, (int)n);
 }
 
+// atan2(y, x) for y > 0 (a crossbridge's angle: y the lattice spacing, at
+// least 1 nm) in double precision.  |x| <= y: pi/2 - atan(x / y); |x| > y:
+// atan(y / |x|), mirrored to pi - atan(y / |x|) for x < 0.  Then atan(t),
+// |t| <= 1: above tan(pi/8), pi/4 + atan((|t| - 1) / (|t| + 1)), so that the
+// odd Taylor series u - u^3/3 + u^5/5 - ... runs on |u| <= tan(pi/8) = 0.4142,
+// where 22 terms leave a truncation below 1e-18 (Horner's rule in u^2).
+// Within 4 ulp of the C library's atan2 over the spacings and offsets of a
+// lattice (tests/test_exp_f64.py).  Coefficients through an opaque pointer on
+// the device, as in exp_f64.
+#define MF_ATAN_TAYLOR                                                                          \
+  {-1.0 / 43.0, 1.0 / 41.0, -1.0 / 39.0, 1.0 / 37.0, -1.0 / 35.0, 1.0 / 33.0, -1.0 / 31.0, 1.0 / 29.0, \
+   -1.0 / 27.0, 1.0 / 25.0, -1.0 / 23.0, 1.0 / 21.0, -1.0 / 19.0, 1.0 / 17.0, -1.0 / 15.0, 1.0 / 13.0, \
+   -1.0 / 11.0, 1.0 / 9.0,  -1.0 / 7.0,  1.0 / 5.0,  -1.0 / 3.0,  1.0}
+static __constant__ double kAtanTaylor[22] = MF_ATAN_TAYLOR;
+__host__ __device__ inline double atan2_pos_f64(double y, double x) {
+  const double ax = fabs(x);
+  const bool near_axis = ax <= y;  // the angle lies in [pi/4, 3pi/4]
+  const double t = near_axis ? x / y : y / ax;
+  const double at = fabs(t);
+  const bool big = at > 0.41421356237309503;
+  const double u = big ? (at - 1.0) / (at + 1.0) : at;
+#ifdef __HIP_DEVICE_COMPILE__
+  const __attribute__((address_space(4))) double* c = (const __attribute__((address_space(4))) double*)kAtanTaylor;
+  asm volatile("" : "+s"(c));
+#else
+  static const double c[22] = MF_ATAN_TAYLOR;
+#endif
+  const double z = u * u;
+  double p = c[0];
+#pragma unroll
+  for (int k = 1; k < 22; ++k) p = fma(p, z, c[k]);
+  double a = fma(u, p, big ? 0.78539816339744831 : 0.0);  // atan(|t|)
+  a = t < 0.0 ? -a : a;                                     // atan(t)
+  if (near_axis) return 1.5707963267948966 - a;
+  return x > 0.0 ? a : 3.1415926535897931 - a;
+}
+
 // Per-instance table layout (floats).
 constexpr int TAB_HDR = 16;                 // pCa, d_kin, d_abs, valid, ...
 constexpr int TAB_TM = TAB_HDR;             // ktm x 27 configs x 4 rows x 4 cumsums
@@ -281,7 +318,8 @@ template <typename T>
 __device__ inline XbRates<T> xb_rates(const float* prm, const XbPre<T>& pre, T d, T x, T perm) {
   const T cap = T(RATE_CAP);
   const T r = sqrt(x * x + d * d);
-  const T th = atan2(d, x);
+  T th;
+  if constexpr (std::is_same<T, double>::value) th = atan2_pos_f64(d, x); else th = atan2(d, x);
   const T ikt = pre.ikt;
   const T dgw = r - T(prm[P_G_REST_W]), dcw = th - T(prm[P_C_REST_W]);
   const T dgs = r - T(prm[P_G_REST_S]), dcs = th - T(prm[P_C_REST_S]);

@@ -862,10 +862,15 @@ __device__ __forceinline__ Springs springs(const LDS float* prm, int st) {
 // residual takes it from a float atan2 (|error| ~1e-7 rad, ~2e-7 pN per
 // head, far below the stopping tolerance at any one node).  The radial
 // residual of dynamic spacing sums over every bound head of the lattice,
-// where that rounding would add up to ~1e-5 pN, so it uses a double atan2.
+// where that rounding would add up to ~1e-5 pN, so it uses a double atan2:
+// atan2_pos_f64 (the spacing is positive), whose coefficients, unlike the
+// library function's, the compiler does not hold in registers across the
+// time loop.
 template <typename T, bool EXACT_ANGLE>
 __device__ __forceinline__ T xb_angle(T d, T xd) {
-  if constexpr (EXACT_ANGLE) return atan2(d, xd); else return T(atan2f((float)d, (float)xd));
+  if constexpr (EXACT_ANGLE && std::is_same<T, double>::value) return atan2_pos_f64(d, xd);
+  else if constexpr (EXACT_ANGLE) return atan2(d, xd);
+  else return T(atan2f((float)d, (float)xd));
 }
 
 // Axial force of a bound XB at offset xd and angle th (forces.py:366-399)

@@ -1,8 +1,12 @@
-// Host-side check of mf::exp_f64 (the step kernel's double exp) against the
-// C library's exp: the largest ulp distance over 2e6 pseudo-random points of
-// [-745, 709.7] and [-100, 100] (normal results), the largest absolute error
-// over the subnormal range, and the special values.  Prints one line:
+// Host-side check of mf::exp_f64 and mf::atan2_pos_f64 (the step kernel's
+// double exp and crossbridge angle) against the C library: for exp the
+// largest ulp distance over 2e6 pseudo-random points of [-708, 709.7] and
+// [-100, 100] (normal results), the largest absolute error over the
+// subnormal range, and the special values; for atan2 the largest ulp
+// distance over 2e6 pairs y in [0.5, 40], x in [-60, 60] and the axis cases.
+// Prints two lines:
 // worst_ulp x_at_worst subnormal_abs_err inf_at_710 nan_in_nan zero_below_745 one_at_zero
+// atan2_worst_ulp y_at_worst x_at_worst exact_axis_cases
 #include "mf_common.h"
 #include <cmath>
 #include <cstdint>
@@ -35,5 +39,24 @@ int main() {
   for (double x = -745.2; x < -708.0; x += 0.001) sub = std::fmax(sub, std::fabs(std::exp(x) - mf::exp_f64(x)));
   std::printf("%lld %.17g %.6g %d %d %d %d\n", (long long)worst, wx, sub, (int)std::isinf(mf::exp_f64(710.0)),
               (int)std::isnan(mf::exp_f64(NAN)), (int)(mf::exp_f64(-800.0) == 0.0), (int)(mf::exp_f64(0.0) == 1.0));
+  int64_t aw = 0;
+  double ay = 0.0, axw = 0.0;
+  for (int i = 0; i < 2000000; ++i) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    const double u = (double)(s >> 11) / 9007199254740992.0;
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    const double v = (double)(s >> 11) / 9007199254740992.0;
+    const double y = 0.5 + 39.5 * u, x = -60.0 + 120.0 * v;
+    const int64_t d = ulps(std::atan2(y, x), mf::atan2_pos_f64(y, x));
+    if (d > aw) { aw = d; ay = y; axw = x; }
+  }
+  const int axis = (mf::atan2_pos_f64(14.0, 0.0) == std::atan2(14.0, 0.0)) +
+                   (mf::atan2_pos_f64(14.0, 14.0) == std::atan2(14.0, 14.0)) +
+                   (mf::atan2_pos_f64(14.0, -14.0) == std::atan2(14.0, -14.0));
+  std::printf("%lld %.17g %.17g %d\n", (long long)aw, ay, axw, axis);
   return 0;
 }
