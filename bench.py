@@ -31,7 +31,10 @@ Other workloads (``--workload``) print one line each:
   thick filament, nu 0);
 * ``sinusoid``: 4x4, 64 frequencies log-spaced 1-1000 Hz, amplitude 2.5 nm
   about z 1000 nm, pCa 4.5, 2000 steps at dt 0.1 ms, with the complex
-  stiffness per frequency.
+  stiffness per frequency;
+* ``lattices``: the step time of each lattice size (``--sizes``), 96
+  instances of one lattice per launch, pCa U[4.5, 6.5], z 900 nm, fixed
+  and / or dynamic spacing (``--spacing``), 30 timed steps after 10.
 """
 from __future__ import annotations
 
@@ -430,6 +433,48 @@ def workload_sinusoid(cfg):
                           "too_short_hz": [round(float(f), 4) for f in freqs[~ok]]}}
 
 
+def workload_lattices(sizes, spacing):
+    """Per-lattice step time: 96 instances of one lattice per launch, pCa
+    U[4.5, 6.5] (default_rng(5)), z 900 nm, spacing 14 nm, refined solver
+    (CG cap 6, forcing 0.003), 10 untimed steps then 30 timed; dynamic
+    spacing with K_lat 5 pN/nm per thick filament, nu 0."""
+    import torch
+    from multifil_jax_amd.topology import SarcTopology
+    from multifil_jax_amd.params import get_skeletal_params
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig
+    s, d = get_skeletal_params()
+    B, W, K = 96, 10, 30
+    c = SolverConfig(mode="refined", n_cg=6, cg_forcing=0.003)
+    pca = np.random.default_rng(5).uniform(4.5, 6.5, B).astype(np.float32)
+    modes = {"fixed": (False,), "dynamic": (True,), "both": (False, True)}[spacing]
+    rows = []
+    for n in sizes:
+        topo = SarcTopology.create(n, n, s, d)
+        for dyn in modes:
+            drv = np.zeros((B, W + K, 3), np.float32)
+            drv[..., 0], drv[..., 2] = 900.0, 14.0
+            drv[..., 1] = pca[:, None]
+            kw = {"dynamic": (np.full(B, 5.0 * topo.n_thick, np.float32), np.zeros(B, np.float32))} if dyn else {}
+            bt = Batch(LatticePack([topo]), np.zeros(B, np.int32), d.rows(B), drv,
+                       seeds=np.arange(B, dtype=np.uint64), **kw)
+            bt.run(W, cfg=c)
+            bt.prepare(K, t0=W)
+            torch.cuda.synchronize()
+            ms = _timed(bt, K, c, t0=W)
+            m = bt.metrics()
+            rows.append({"lattice": f"{n}x{n}", "spacing": "dynamic" if dyn else "fixed", "block": bt.block,
+                         "us_per_step": round(1e3 * ms / K, 2),
+                         "newton_mean": round(float(m[:, W:, 51].mean()), 3),
+                         "max_solver_residual_pN": float(m[:, W:, 1].max())})
+            print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    return {"metric": "us per step of 96 instances, per lattice", "unit": "us", "higher_is_better": False,
+            "n_gpus": 1, "steps": K, "data": "synthetic",
+            "config": {"workload": "lattices: 96 instances of one lattice per launch, pCa U[4.5,6.5], z 900 nm, "
+                                   "spacing 14 nm (dynamic: K_lat 5 pN/nm per thick, nu 0), dt 1 ms, "
+                                   "10 + 30 steps", "solver": "refined", "n_cg": 6, "cg_forcing": 0.003},
+            "results": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +486,10 @@ def main():
                     help="threads per workgroup (default: the engine's choice for the lattice)")
     ap.add_argument("--n-cg", type=int, default=6, help="CG iterations per Newton step")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/oracle/cpu legs")
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "plain", "uneven", "sinusoid"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "plain", "uneven", "sinusoid", "lattices"])
+    ap.add_argument("--sizes", default="2,3,4,6", help="lattices workload: n of the n x n lattices")
+    ap.add_argument("--spacing", default="both", choices=["fixed", "dynamic", "both"],
+                    help="lattices workload: spacing modes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the block-size / CG-cap autotuner (sweep and uneven workloads)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -457,7 +505,9 @@ def main():
     if args.workload != "sweep":
         fn = {"plain": workload_plain,
               "uneven": lambda c: workload_uneven(c, args.block, not args.no_autotune),
-              "sinusoid": workload_sinusoid}[args.workload]
+              "sinusoid": workload_sinusoid,
+              "lattices": lambda c: workload_lattices([int(x) for x in args.sizes.split(",")], args.spacing)}[
+                  args.workload]
         print(json.dumps(fn(cfg)))
         return
     B, K, W = args.batch, args.steps, args.warmup
