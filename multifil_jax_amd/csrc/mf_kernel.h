@@ -485,6 +485,8 @@ struct Slots {
   int fil[MS];    // filament index within its type
   bool thick[MS];
   int nch;        // chains owned by this wave
+  int per0;       // slots of the wave's first chain (slots 0..per0-1)
+  bool thick0;    // the first chain is a thick filament
 };
 
 // XB partners of a lane's slot nodes, fixed during a solve: for a crown the
@@ -777,6 +779,33 @@ __device__ __forceinline__ float wg_max(Red<BS>& rd, float v, int lane, int wave
 template <int MS>
 __device__ __forceinline__ void chain_scan(float (&v)[MS], const Slots<MS>& sl, int lane, int thick_dir,
                                            int thin_dir) {
+  // (two-slot kernels only: in the many-slot ones the second copy of the
+  // scan costs the CG loop registers, 6x6 steps 25 % slower)
+  if (MS <= 2 && sl.nch == 1) {
+    // one chain in the wave, in slots 0..per0-1: the same operations as
+    // the general case below without its per-slot chain tests
+    const int dir = sl.thick0 ? thick_dir : thin_dir;
+    float acc = 0.0f;
+    if (dir == 0) {
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < sl.per0) { acc += v[s]; v[s] = acc; }
+      const float ex = wave_prefix_excl(wave_prefix(acc, lane), lane);
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < sl.per0) v[s] += ex;
+    } else {
+#pragma unroll
+      for (int s = MS - 1; s >= 0; --s)
+        if (s < sl.per0) { acc += v[s]; v[s] = acc; }
+      const float incl = wave_prefix(acc, lane);
+      const float ex = read_lane(incl, 63) - incl;  // sum over the lanes after this one
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < sl.per0) v[s] += ex;
+    }
+    return;
+  }
   for (int k = 0; k < sl.nch; ++k) {
     bool th = false;
 #pragma unroll
@@ -1699,6 +1728,14 @@ __global__ __launch_bounds__(BS) void run_kernel(const mf_launch L) {
       base += per;
     }
     sl.nch = k;
+    // the first chain's shape, uniform per wave (readfirstlane: scalar
+    // registers, so the single-chain scan below branches on them for free)
+    {
+      const int c0 = wave < dm.nch ? wave : 0;
+      const bool th0 = c0 < dm.Tk;
+      sl.per0 = __builtin_amdgcn_readfirstlane(((th0 ? dm.nc : dm.ns) + WAVE - 1) / WAVE);
+      sl.thick0 = __builtin_amdgcn_readfirstlane(th0 ? 1 : 0) != 0;
+    }
   }
   // per-slot rests: thick rA = rest_i, rB = rest_{i+1}; thin rA = rest_i, rB = rest_{i-1}
   float rA[MS], rB[MS];
