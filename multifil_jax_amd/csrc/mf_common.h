@@ -467,19 +467,27 @@ __device__ inline void expm_ref(T A[N][N]) {
       W[i][j] = V[i][j] - A2[i][j];
       A[i][j] = V[i][j] + A2[i][j];
     }
+  // Gauss-Jordan with the pivot rows left unscaled until the end: step k
+  // removes column k from every other row with the multiplier W[i][k] /
+  // W[k][k]; then each row is scaled by the reciprocal of its pivot (as it
+  // was at its own step)
+  T ipiv[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const T inv = T(1) / W[k][k];
-#pragma unroll
-    for (int j = 0; j < N; ++j) { W[k][j] *= inv; A[k][j] *= inv; }
+    ipiv[k] = inv;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       if (i == k) continue;
-      const T f = W[i][k];
+      const T f = W[i][k] * inv;
 #pragma unroll
       for (int j = 0; j < N; ++j) { W[i][j] = fma(-f, W[k][j], W[i][j]); A[i][j] = fma(-f, A[k][j], A[i][j]); }
     }
   }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) A[i][j] *= ipiv[i];
   const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
   for (int q = 0; q < nsq; ++q) {
     matmul<N, T>(A, A, A2);
@@ -588,8 +596,10 @@ __device__ __forceinline__ void absorbing_block(const XbRates<T>& k, T dt, T w, 
 // 2 min(l, 2) and 2 min(l, 2) + 1 (lane 3 repeats lane 2's rows), and a
 // matrix product takes the other lanes' rows by DPP quad broadcasts.  Every
 // row sees the operations of expm_ref in the same order (products with k
-// ascending, the same Gauss-Jordan steps), so the result is bit for bit
-// expm_ref's.  All four lanes of the quad must be active.
+// ascending, the same Gauss-Jordan steps; the pivot row's own update, a
+// multiply-add with multiplier 0, leaves every entry but a -0 unchanged),
+// so the result is expm_ref's bit for bit up to the sign of a zero entry.
+// All four lanes of the quad must be active.
 // ---------------------------------------------------------------------------
 template <int Q>
 __device__ __forceinline__ double quad_bcast_d(double v) {
@@ -677,8 +687,11 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
       W[i][j] = V[i][j] - A2[i][j];
       A[i][j] = V[i][j] + A2[i][j];
     }
-  // Gauss-Jordan: every lane scales the broadcast pivot row itself (the
-  // owner's arithmetic exactly), the owner keeps the scaled row
+  // Gauss-Jordan as expm_ref's, pivot rows unscaled until the end: the
+  // pivot row's own lane takes multiplier 0 (a select on the multiplier,
+  // not on every entry; fma(-0, p, w) returns w) and keeps the pivot's
+  // reciprocal for the final scaling
+  double ipiv[2] = {0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     double pw[6], pa[6];
@@ -692,22 +705,22 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
     }
     const double inv = 1.0 / pw[k];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) { pw[j] *= inv; pa[j] *= inv; }
-    // selects, not a branch: the owner lane of a row differs within the
-    // quad, and a lane-divergent branch here costs the register file dearly
-#pragma unroll
     for (int i = 0; i < 2; ++i) {
       const bool own = 2 * rp + i == k;
-      const double f = W[i][k];
+      ipiv[i] = own ? inv : ipiv[i];
+      const double f = own ? 0.0 : W[i][k] * inv;
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        const double w = fma(-f, pw[j], W[i][j]), a = fma(-f, pa[j], A[i][j]);
-        W[i][j] = own ? pw[j] : w;
-        A[i][j] = own ? pa[j] : a;
+        W[i][j] = fma(-f, pw[j], W[i][j]);
+        A[i][j] = fma(-f, pa[j], A[i][j]);
       }
     }
     __builtin_amdgcn_sched_barrier(0);
   }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= ipiv[i];
   const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
   // squarings in pairs, A -> A2 -> A, so no copy between them
   int q = 0;
@@ -741,8 +754,9 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
 // h = lane & 1 owns rows 3h..3h+2.  Row k of a product's right operand comes
 // from lane k / 3 of the pair by one DPP quad permutation ([0,0,2,2] or
 // [1,1,3,3]; the owner reads its own value through it), so every row again
-// sees expm_ref's operations in expm_ref's order and the result is bit for
-// bit expm_ref's.  Against the quad form a key costs 2 lanes instead of 4
+// sees expm_ref's operations in expm_ref's order and the result is
+// expm_ref's bit for bit up to the sign of a zero entry (as the quad's).
+// Against the quad form a key costs 2 lanes instead of 4
 // (the quad's fourth lane repeats the third's rows) and a product 72 DPP
 // moves and 108 FMAs per lane instead of 72 and 72: half as many
 // sub-rounds, each about a quarter longer, for the price of 36 VGPRs per
@@ -821,8 +835,8 @@ __device__ inline void expm_pair6(double A[3][6], int h) {
       W[i][j] = V[i][j] - A2[i][j];
       A[i][j] = V[i][j] + A2[i][j];
     }
-  // Gauss-Jordan as in expm_quad6: every lane scales the broadcast pivot
-  // row itself, the owner keeps the scaled row
+  // Gauss-Jordan as in expm_quad6 (pivot rows unscaled until the end)
+  double ipiv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     double pw[6], pa[6];
@@ -833,20 +847,22 @@ __device__ inline void expm_pair6(double A[3][6], int h) {
     }
     const double inv = 1.0 / pw[k];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) { pw[j] *= inv; pa[j] *= inv; }
-#pragma unroll
     for (int i = 0; i < 3; ++i) {
       const bool own = 3 * h + i == k;
-      const double f = W[i][k];
+      ipiv[i] = own ? inv : ipiv[i];
+      const double f = own ? 0.0 : W[i][k] * inv;
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        const double w = fma(-f, pw[j], W[i][j]), a = fma(-f, pa[j], A[i][j]);
-        W[i][j] = own ? pw[j] : w;
-        A[i][j] = own ? pa[j] : a;
+        W[i][j] = fma(-f, pw[j], W[i][j]);
+        A[i][j] = fma(-f, pa[j], A[i][j]);
       }
     }
     __builtin_amdgcn_sched_barrier(0);
   }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] *= ipiv[i];
   const int nsq = s < N_SQUARINGS_CAP ? s : N_SQUARINGS_CAP;
   int q = 0;
   for (; q + 2 <= nsq; q += 2) {

@@ -225,8 +225,9 @@ __device__ __forceinline__ void xb_table_entry(const Pops& P, double pCa, double
 }
 
 // The rows of one XB key's entry from the quad's generator rows A (this
-// lane's two rows, rp = min(l, 2)): scaled by dt, expm_quad6 (bit for bit
-// xb_table_entry's result), stored; l = lane & 3.
+// lane's two rows, rp = min(l, 2)): scaled by dt, expm_quad6
+// (xb_table_entry's result, bit for bit up to the sign of a zero), stored;
+// l = lane & 3.
 __device__ __forceinline__ void xb_quad_finish(double A[2][6], double dt, int key, bool absorbing, float* xbrows,
                                                float* absrows, int l) {
   const int rp = l < 2 ? l : 2;
