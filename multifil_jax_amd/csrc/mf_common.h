@@ -459,7 +459,7 @@ __device__ inline void expm_ref(T A[N][N]) {
       V[i][j] = b0 * id + b2 * A2[i][j] + b4 * A4[i][j] + b6 * V[i][j];
       W[i][j] = b1 * id + b3 * A2[i][j] + b5 * A4[i][j];
     }
-  matmul<N, T>(A, W, A2);  // U
+  matmul<N, T>(W, A, A2);  // U = W A (= A W: W is a polynomial in A)
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -609,30 +609,52 @@ __device__ __forceinline__ double quad_bcast_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// Entry (k, j) of the XB generator can be nonzero (xb_gen_row): row 0 at
+// columns 0, 1, 4, 5; row 1 at 0-2; row 2 at 1-3; row 3 at 2-4; row 4 at 0
+// and 4; row 5 at 0 and 5.  A product whose right operand has this pattern
+// (SPB: the scaled generator) skips the other entries' broadcasts and
+// multiply-adds, which leave the sums unchanged (but for the sign of a zero).
+constexpr bool xb_gen_nz(int k, int j) {
+  return k == 0 ? (j == 0 || j == 1 || j == 4 || j == 5)
+       : k == 1 ? (j <= 2)
+       : k == 2 ? (j >= 1 && j <= 3)
+       : k == 3 ? (j >= 2 && j <= 4)
+       : k == 4 ? (j == 0 || j == 4)
+                : (j == 0 || j == 5);
+}
+
 // C = A B for the row pair of this lane (A, C: this lane's rows; B: the
 // lane's rows of B).  Row k of B comes from quad lane k / 2, one row at a
 // time.  C may alias A or B only through the caller's copy.
-template <int K>
+template <int K, bool SPB>
 __device__ __forceinline__ void quad_fma_row(const double A[2][6], const double B[2][6], double acc[2][6]) {
   double b[6];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) b[j] = quad_bcast_d<K / 2>(B[K & 1][j]);
+  for (int j = 0; j < 6; ++j)
+    if (!SPB || xb_gen_nz(K, j)) b[j] = quad_bcast_d<K / 2>(B[K & 1][j]);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
+    for (int j = 0; j < 6; ++j) {
+      if (SPB && !xb_gen_nz(K, j)) {
+        if (K == 0) acc[i][j] = 0.0;
+      } else {
+        acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
+      }
+    }
   // keep the scheduler from hoisting the next row's broadcasts above these
   // FMAs: all six rows in flight at once would not fit the register file
   __builtin_amdgcn_sched_barrier(0);
 }
+template <bool SPB = false>
 __device__ __forceinline__ void quad_matmul6(const double A[2][6], const double B[2][6], double C[2][6]) {
   double acc[2][6];
-  quad_fma_row<0>(A, B, acc);
-  quad_fma_row<1>(A, B, acc);
-  quad_fma_row<2>(A, B, acc);
-  quad_fma_row<3>(A, B, acc);
-  quad_fma_row<4>(A, B, acc);
-  quad_fma_row<5>(A, B, acc);
+  quad_fma_row<0, SPB>(A, B, acc);
+  quad_fma_row<1, SPB>(A, B, acc);
+  quad_fma_row<2, SPB>(A, B, acc);
+  quad_fma_row<3, SPB>(A, B, acc);
+  quad_fma_row<4, SPB>(A, B, acc);
+  quad_fma_row<5, SPB>(A, B, acc);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -662,7 +684,7 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
   const double b0 = 1.0, b1 = 1.0 / 2.0, b2 = 1.0 / 9.0, b3 = 1.0 / 72.0, b4 = 1.0 / 1008.0,
                b5 = 1.0 / 30240.0, b6 = 1.0 / 1209600.0;
   double A2[2][6], A4[2][6], A6[2][6];
-  quad_matmul6(A, A, A2);
+  quad_matmul6<true>(A, A, A2);
   quad_matmul6(A2, A2, A4);
   quad_matmul6(A4, A2, A6);
   // V over A6 and W over A4, entry by entry (each read once, before it is
@@ -679,7 +701,7 @@ __device__ inline void expm_quad6(double A[2][6], int l) {
     }
   double (&V)[2][6] = A6;
   double (&W)[2][6] = A4;
-  quad_matmul6(A, W, A2);  // U
+  quad_matmul6<true>(W, A, A2);  // U = W A (= A W: W is a polynomial in A)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -770,25 +792,34 @@ __device__ __forceinline__ double pair_bcast_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
-template <int K>
+template <int K, bool SPB>
 __device__ __forceinline__ void pair_fma_row(const double A[3][6], const double B[3][6], double acc[3][6]) {
   double b[6];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) b[j] = pair_bcast_d<K / 3>(B[K % 3][j]);
+  for (int j = 0; j < 6; ++j)
+    if (!SPB || xb_gen_nz(K, j)) b[j] = pair_bcast_d<K / 3>(B[K % 3][j]);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
+    for (int j = 0; j < 6; ++j) {
+      if (SPB && !xb_gen_nz(K, j)) {
+        if (K == 0) acc[i][j] = 0.0;
+      } else {
+        acc[i][j] = K == 0 ? A[i][0] * b[j] : fma(A[i][K], b[j], acc[i][j]);
+      }
+    }
   __builtin_amdgcn_sched_barrier(0);
 }
-// C = A B for the lane's three rows; C must not alias A or B.
+// C = A B for the lane's three rows; C must not alias A or B.  SPB: B has
+// the XB generator's zero pattern.
+template <bool SPB = false>
 __device__ __forceinline__ void pair_matmul6(const double A[3][6], const double B[3][6], double C[3][6]) {
-  pair_fma_row<0>(A, B, C);
-  pair_fma_row<1>(A, B, C);
-  pair_fma_row<2>(A, B, C);
-  pair_fma_row<3>(A, B, C);
-  pair_fma_row<4>(A, B, C);
-  pair_fma_row<5>(A, B, C);
+  pair_fma_row<0, SPB>(A, B, C);
+  pair_fma_row<1, SPB>(A, B, C);
+  pair_fma_row<2, SPB>(A, B, C);
+  pair_fma_row<3, SPB>(A, B, C);
+  pair_fma_row<4, SPB>(A, B, C);
+  pair_fma_row<5, SPB>(A, B, C);
 }
 
 // A <- expm(A) for a 6x6 matrix held as row triples by a pair; h = lane & 1.
@@ -812,7 +843,7 @@ __device__ inline void expm_pair6(double A[3][6], int h) {
   const double b0 = 1.0, b1 = 1.0 / 2.0, b2 = 1.0 / 9.0, b3 = 1.0 / 72.0, b4 = 1.0 / 1008.0,
                b5 = 1.0 / 30240.0, b6 = 1.0 / 1209600.0;
   double A2[3][6], A4[3][6], A6[3][6];
-  pair_matmul6(A, A, A2);
+  pair_matmul6<true>(A, A, A2);
   pair_matmul6(A2, A2, A4);
   pair_matmul6(A4, A2, A6);
 #pragma unroll
@@ -827,7 +858,7 @@ __device__ inline void expm_pair6(double A[3][6], int h) {
     }
   double (&V)[3][6] = A6;
   double (&W)[3][6] = A4;
-  pair_matmul6(A, W, A2);  // U
+  pair_matmul6<true>(W, A, A2);  // U = W A
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
