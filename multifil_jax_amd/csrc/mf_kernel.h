@@ -432,6 +432,21 @@ __device__ __attribute__((noinline)) void xb_table_rebuild(const Pops P, float p
                                              nthreads, list, count, order, hist, pre);
 }
 
+// The absorbing rows alone, one lane per key, out of line in a function of
+// their own: its few registers are all the call saves and restores (the
+// sampling-row rebuild above uses the whole register file).
+template <int BS>
+__device__ __attribute__((noinline)) void absorbing_rebuild(const Pops P, float pCa, double d, float dt,
+                                                            const float* centers, int nb, int kxb, float* absrows,
+                                                            int tid, const unsigned short* list, int count,
+                                                            const double* pre) {
+  const int n = list ? count : kxb * 2 * nb;
+  for (int i = tid; i < n; i += BS) {
+    const int key = list ? list[i] : i;
+    absorbing_entry(P, pre, (double)pCa, d, (double)dt, (double)centers[key % nb], key, nb, absrows);
+  }
+}
+
 // The step kernel's rebuild: a call for the LDS-resident variants, inline
 // for the global-workspace ones.  Values live across a call sit in the
 // callee-saved registers or are spilled, and at 12 node slots per lane the
@@ -445,6 +460,12 @@ __device__ __forceinline__ void rebuild_rows(const Pops& P, float pCa, double d,
                                              int nb, int kxb, bool absorbing, float* xbrows, float* absrows,
                                              int tid, const unsigned short* list, int count,
                                              unsigned short* order, uint32_t* hist, const double* pre) {
+  if constexpr (GW == 0) {
+    if (absorbing && !P.blend_xb) {
+      absorbing_rebuild<BS>(P, pCa, d, dt, centers, nb, kxb, absrows, tid, list, count, pre);
+      return;
+    }
+  }
   if constexpr (GW > 0)
     xb_table_rebuild_body<(BS <= 512 ? 2 : 4)>(P, pCa, d, dt, centers, nb, kxb, absorbing, xbrows, absrows, tid,
                                                BS, list, count, order, hist, pre);
