@@ -111,9 +111,11 @@ def test_subpopulation_parity(torch_cuda, topo22, skeletal, pdict, mode, scales)
     assert np.all(mg[:, 1] <= 1e-5)
 
 
-def test_explicit_subpopulation_with_dynamic_spacing(torch_cuda, topo22, skeletal, pdict):
-    """c_zone labels (XB and TM) on a dynamic-spacing run: the per-label
-    tables are rebuilt in the kernel at the carried and at the solved
+@pytest.mark.parametrize("mode", ["c_zone", "mean_field"])
+def test_subpopulation_with_dynamic_spacing(torch_cuda, topo22, skeletal, pdict, mode):
+    """Subpopulations on a dynamic-spacing run: c_zone labels (XB and TM),
+    whose per-label tables, or a mean-field fraction of 0.4, whose blended
+    generators, are rebuilt in the kernel at the carried and at the solved
     spacing each step; parity against the oracle as above, plus the solved
     spacing within 1e-6 nm."""
     from multifil_jax_amd import Subpopulation, run
@@ -121,8 +123,10 @@ def test_explicit_subpopulation_with_dynamic_spacing(torch_cuda, topo22, skeleta
     from multifil_jax_amd.subpopulation import population_params
     from multifil_jax_amd.params import DYNAMIC_FIELDS
     s, d = skeletal
-    sp = Subpopulation.c_zone(topo22, **SCALES)
+    sp = (Subpopulation.c_zone(topo22, **SCALES) if mode == "c_zone"
+          else Subpopulation.mean_field(0.4, **SCALES))
     T, K = 12, 5.0
+    print(f"subpopulation {mode} with dynamic spacing", flush=True)
     res = run(topo22, duration_ms=T, pCa=4.5, z_line=950.0, rng_seed=3, solver="refined",
               static_params=s, dynamic_params=d, subpopulation=sp, K_lat=K)
     plan = plan_sweep(topo22, duration_ms=T, pCa=4.5, z_line=950.0, dynamic_params=d, subpopulation=sp,
@@ -131,7 +135,8 @@ def test_explicit_subpopulation_with_dynamic_spacing(torch_cuda, topo22, skeleta
     P = population_params(plan.params, rp)[0]
     pops = [{n: float(P[k, i]) for i, n in enumerate(DYNAMIC_FIELDS)} for k in range(rp.K)]
     pop = mo.Pop(rp.mode, pops, rp.fractions[0], has_xb=rp.has_xb, has_tm=rp.has_tm,
-                 xb_label=rp.xb_mask[0], tm_label=rp.tm_mask[0])
+                 xb_label=None if rp.xb_mask is None else rp.xb_mask[0],
+                 tm_label=None if rp.tm_mask is None else rp.tm_mask[0])
     drv = plan.drivers[0]
     _, out = mo.run(topo22, pdict, drv[:, 0], drv[:, 1], drv[:, 2], 1.0, (3 << 32) | 0,
                     mo.SolverConfig(mode="exact"), K_lat=float(plan.K_lat[0]), nu=0.0, pop=pop)
