@@ -149,3 +149,33 @@ def test_in_kernel_rebuild_equals_split_launches(torch_cuda, topo22, skeletal, d
         for k in a:
             assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
     assert out[0][1][0, :, 5].max() > 0          # the twitch binds heads
+
+
+@pytest.mark.parametrize("mode", ["mean_field", "c_zone"])
+def test_in_kernel_rebuild_with_subpopulations_equals_split_launches(torch_cuda, topo22, skeletal, mode):
+    """As above with subpopulations on a pCa twitch (fixed spacing): a
+    mean-field fraction, whose blended TM and XB rows, or c_zone labels,
+    whose per-label row sets, each instance rebuilds in the kernel when its
+    pCa changes; the same bits as one launch per constant stretch."""
+    from multifil_jax_amd import _lib, Subpopulation
+    from multifil_jax_amd.engine import LatticePack, Batch, SolverConfig, Populations
+    from multifil_jax_amd.simulation import plan_sweep, _populations
+    _, d = skeletal
+    sc = dict(xb_r01_coeff=3.0, tm_k_01=0.5)
+    sp = Subpopulation.c_zone(topo22, **sc) if mode == "c_zone" else Subpopulation.mean_field(0.4, **sc)
+    T = 60
+    plan = plan_sweep(topo22, duration_ms=T, pCa=[4.8, 5.2], z_line=1000.0, dynamic_params=d, subpopulation=sp)
+    pops = _populations(plan, Populations)
+    drv = np.array(plan.drivers, np.float32)
+    drv[0, :, 1] = _twitch(T + 20)[20:]
+    out = []
+    for split in (False, True):
+        print(f"twitch with {mode} subpopulations, split={split}", flush=True)
+        bt = Batch(LatticePack([topo22]), np.zeros(2, np.int32), plan.params, drv,
+                   seeds=np.arange(2, dtype=np.uint64), pops=pops)
+        bt.run(T, cfg=SolverConfig(mode="refined"), split=split,
+               launch_flags=_lib.MF_LAUNCH_DYNAMIC if split else 0)
+        out.append((bt.force(), bt.metrics()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1], equal_nan=True)
+    assert out[0][1][0, :, 5].max() > 0
