@@ -51,8 +51,12 @@ def _cfg(mode):
     return SolverConfig(mode=mode)
 
 
-def test_dynamic_free_run_refined(torch_cuda, topo22, skeletal, pdict):
+@pytest.mark.parametrize("block", [None, 768, 1024])
+def test_dynamic_free_run_refined(torch_cuda, topo22, skeletal, pdict, block):
+    """Default block (512 threads: three node slots per lane, the in-kernel
+    rebuild by lane pairs) and 768 / 1024 threads (two slots, lane quads)."""
     _, d = skeletal
+    print(f"2x2 dynamic free run, block={block}", flush=True)
     T = 20
     K = K_LAT * topo22.n_thick
     # instance 0: constant Z-line, nu = 0; instance 1: a 10 nm ramp with nu = 0.5
@@ -60,7 +64,8 @@ def test_dynamic_free_run_refined(torch_cuda, topo22, skeletal, pdict):
     drv[1, :, 0] = np.linspace(1000.0, 1010.0, T, dtype=np.float32)
     nu = np.array([0.0, 0.5], np.float32)
     bt = _batch([topo22], [0, 0], d, drv, [7, 8], dynamic=(np.full(2, K, np.float32), nu))
-    bt.run(T, cfg=_cfg("refined"))
+    bt.run(T, cfg=_cfg("refined"), block=block)
+    assert bt.block == (block or 512)
     fg, mg = bt.force(), bt.metrics()
     for b in range(2):
         _, out = mo.run(topo22, pdict, drv[b, :, 0], drv[b, :, 1], drv[b, :, 2], 1.0, 7 + b,
