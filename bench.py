@@ -457,7 +457,12 @@ def workload_lattices(sizes, spacing):
             kw = {"dynamic": (np.full(B, 5.0 * topo.n_thick, np.float32), np.zeros(B, np.float32))} if dyn else {}
             bt = Batch(LatticePack([topo]), np.zeros(B, np.int32), d.rows(B), drv,
                        seeds=np.arange(B, dtype=np.uint64), **kw)
-            bt.run(W, cfg=c)
+            try:
+                bt.run(W, cfg=c)
+            except RuntimeError as e:   # a lattice / spacing the engine refuses (before any launch)
+                rows.append({"lattice": f"{n}x{n}", "spacing": "dynamic" if dyn else "fixed", "refused": str(e)})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+                continue
             bt.prepare(K, t0=W)
             torch.cuda.synchronize()
             ms = _timed(bt, K, c, t0=W)

@@ -152,6 +152,17 @@ int pick(const mf_launch* L, const mf_lattice* lats, int n_lats, const Variant**
       return 0;
     }
   }
+  if (dyn) {
+    // the dynamic-spacing variants stop at 24 node slots per lane (8x8)
+    for (int g = 0; g < n_lats; ++g) {
+      int ms, mx;
+      needs(dims_of(lats[g], *L), 1024, ms, mx);
+      if (ms > 24)
+        return fail("dynamic spacing is not available for this lattice (%d node slots per lane at 1024 "
+                    "threads; the dynamic-spacing kernels go up to 24, 8x8): the 32-slot form (10x10) is "
+                    "switched off, since its one run stopped responding for a reason not yet found", ms);
+    }
+  }
   return fail("no kernel variant fits this lattice at block %d (LDS or register slots exceeded)", L->block);
 }
 
